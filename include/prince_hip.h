@@ -1,0 +1,111 @@
+/*
+ * prince_hip.h -- C ABI of the MI355X (gfx950) ETD2 propagation library.
+ *
+ * Every entry point returns 0 on success and a non-zero status on failure;
+ * prh_last_error() then describes the failure (thread-local string).  No
+ * C++ exception crosses this boundary.  All pointers to device memory are
+ * owned by the caller (the Python host holds them as torch tensors); the
+ * library never allocates or frees device memory and keeps no pointer
+ * beyond the call, mirroring the caller-owned-buffer rule of the
+ * reference's native sparse handle (src/prince_cr/mkl_sparse.py:12-14,
+ * 106-108).  Work is enqueued on the given stream and is asynchronous.
+ *
+ * Data layout in device memory (one GPU, FP64 throughout):
+ *   state  x, a, fphi : [nspec][K][d]  (species block, right-hand side,
+ *                       energy bin); for K = 1 this is the reference
+ *                       state vector order princeidx*d + i.
+ *   mvals             : [nwin][mvals_stride]; per rate-cache window the
+ *                       off-diagonal coupling values in block-band layout
+ *                       (zslots doubles) followed by the diagonal (n).
+ *   blocks            : [nblocks][4] = {mother species, first offset
+ *                       o_lo = j - i, number of offsets nslot, value
+ *                       offset voff}; block value (s, i) at voff + s*d + i
+ *                       couples row (da, i) to column (mo, i + o_lo + s).
+ *   items             : [nitems][6] = {da, blk_begin, blk_end, part,
+ *                       nparts, part_base}; one wave per (item, rhs).
+ */
+#ifndef PRINCE_HIP_H
+#define PRINCE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct prh_operator {
+  int32_t nspec;   /* species blocks */
+  int32_t d;       /* energy bins per block (<= 64) */
+  int32_t K;       /* right-hand sides */
+  int32_t nitems;  /* work items per stage */
+  const int32_t* items;
+  const int32_t* blocks;
+  int64_t mvals_stride; /* doubles per window in mvals */
+  int64_t zslots;       /* offset of the diagonal inside a window */
+  const double* mvals;
+  const double* egrid;  /* [d] bin-centre energies (adiabatic loss) */
+  const double* dcoef;  /* [d][7] FD coefficients for offsets -3..3 */
+  const double* ddiag;  /* [d] FD diagonal */
+  const double* pscale; /* [nspec] pair-production scale per species */
+  const double* rpair;  /* [nwin][d] pair-production rate per window */
+  const int32_t* src_slot; /* [nspec] source slot or -1 */
+  int32_t nsrc;            /* sources per right-hand side (J) */
+  int32_t flags;           /* PRH_F_* */
+  const double* src_grid;  /* [nslot][J][K][d] injection grids */
+} prh_operator;
+
+#define PRH_F_PARTIAL_DIFF 1 /* continuous-loss stencil term on */
+#define PRH_F_INJECTION 2    /* source term b(z) on */
+
+typedef struct prh_schedule {
+  int32_t nsteps;
+  int32_t pad_;
+  const double* h;        /* [nsteps] z step (negative) */
+  const double* dldz;     /* [nsteps] dl/dz at the step start */
+  const double* hadia;    /* [nsteps] H(z) * cm2sec (adiabatic kappa / E) */
+  const double* finj;     /* [nsteps] dl/dz * cm2sec (injection factor) */
+  const int32_t* win;     /* [nsteps] rate-cache window of the step */
+  const double* src_coef; /* [nsteps][J][K] norm * evolution(z) per source */
+} prh_schedule;
+
+typedef struct prh_state {
+  double* x;        /* [nspec][K][d] solution, updated in place */
+  double* a;        /* [nspec][K][d] stage-1 vector */
+  double* fphi;     /* [nspec][K][d] F(x) of stage 1 */
+  double* partials; /* [nparts_total][K][d] partial row sums */
+  int32_t* counters;/* [nitems][K], zero on entry, left zero on exit */
+} prh_state;
+
+/* Library version and last error (thread-local, never NULL). */
+int32_t prh_version(void);
+const char* prh_last_error(void);
+
+/* Rate fold for many photon vectors at once (FP64 MFMA GEMM):
+ *   out[w*out_stride + r] = sum_j phot[w*dph + j] * kern[r*dph + j]
+ * for w < nwin, r < nrows.  Replaces the per-window dense fold
+ * M.data = B @ n_gamma of src/prince_cr/interaction_rates.py:1051-1061
+ * (host) and :930-964 (GPU), and the split fold of :824-847, evaluated for
+ * every refresh window of a solve in one launch. */
+int32_t prh_fold_gemm(const double* kern, int64_t nrows, int32_t dph, const double* phot,
+                      int32_t nwin, double* out, int64_t out_stride, void* stream);
+
+/* Row gather dst[r] = sel[r] >= 0 ? src[sel[r]] : 0 for rows of ncols
+ * doubles.  Builds the split fold kernel of
+ * src/prince_cr/interaction_rates.py:780-822 (rows of B that feed M_off and
+ * the diagonal) directly in block-band order. */
+int32_t prh_gather_rows(const double* src, const int64_t* sel, int64_t nrows, int32_t ncols,
+                        double* dst, void* stream);
+
+/* ETD2 steps k_begin <= k < k_end on the device (two fused stage kernels
+ * per step).  Replaces the step loop of src/prince_cr/solvers/etd2.py:478-517
+ * with the step body of etd2.py:412-439 / 442-475 and the operator of
+ * src/prince_cr/solvers/propagation.py:1360-1508 (apply_F, diagonal d,
+ * kappa = kappa_adia + kappa_pair, injection b). */
+int32_t prh_etd2_steps(const prh_operator* op, const prh_schedule* sch, const prh_state* st,
+                       int32_t k_begin, int32_t k_end, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRINCE_HIP_H */

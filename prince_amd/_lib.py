@@ -1,0 +1,117 @@
+"""ctypes binding of ``libprince_hip.so`` (C ABI declared in include/prince_hip.h).
+
+The library is built in-tree next to this file (``prince_amd/build.py``).
+Loading fails loudly when it is missing: there is no fallback path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libprince_hip.so")
+
+EXPORTED = (
+    "prh_version",
+    "prh_last_error",
+    "prh_fold_gemm",
+    "prh_gather_rows",
+    "prh_etd2_steps",
+)
+
+PRH_F_PARTIAL_DIFF = 1
+PRH_F_INJECTION = 2
+
+_p = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+
+
+class PrhOperator(ctypes.Structure):
+    _fields_ = [
+        ("nspec", _i32),
+        ("d", _i32),
+        ("K", _i32),
+        ("nitems", _i32),
+        ("items", _p),
+        ("blocks", _p),
+        ("mvals_stride", _i64),
+        ("zslots", _i64),
+        ("mvals", _p),
+        ("egrid", _p),
+        ("dcoef", _p),
+        ("ddiag", _p),
+        ("pscale", _p),
+        ("rpair", _p),
+        ("src_slot", _p),
+        ("nsrc", _i32),
+        ("flags", _i32),
+        ("src_grid", _p),
+    ]
+
+
+class PrhSchedule(ctypes.Structure):
+    _fields_ = [
+        ("nsteps", _i32),
+        ("pad_", _i32),
+        ("h", _p),
+        ("dldz", _p),
+        ("hadia", _p),
+        ("finj", _p),
+        ("win", _p),
+        ("src_coef", _p),
+    ]
+
+
+class PrhState(ctypes.Structure):
+    _fields_ = [
+        ("x", _p),
+        ("a", _p),
+        ("fphi", _p),
+        ("partials", _p),
+        ("counters", _p),
+    ]
+
+
+_LIB = None
+
+
+class HipLibraryError(RuntimeError):
+    pass
+
+
+def load():
+    """Load (once) and return the ctypes library handle."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise HipLibraryError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `python -m prince_amd.build`; the HIP backend has no fallback."
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.prh_version.restype = _i32
+    lib.prh_last_error.restype = ctypes.c_char_p
+    lib.prh_fold_gemm.restype = _i32
+    lib.prh_fold_gemm.argtypes = [_p, _i64, _i32, _p, _i32, _p, _i64, _p]
+    lib.prh_gather_rows.restype = _i32
+    lib.prh_gather_rows.argtypes = [_p, _p, _i64, _i32, _p, _p]
+    lib.prh_etd2_steps.restype = _i32
+    lib.prh_etd2_steps.argtypes = [
+        ctypes.POINTER(PrhOperator),
+        ctypes.POINTER(PrhSchedule),
+        ctypes.POINTER(PrhState),
+        _i32,
+        _i32,
+        _p,
+    ]
+    _LIB = lib
+    return lib
+
+
+def check(status, what):
+    if status != 0:
+        msg = load().prh_last_error().decode(errors="replace")
+        raise HipLibraryError(f"{what} failed: {msg}")
