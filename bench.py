@@ -1,0 +1,275 @@
+"""Benchmark of the MI355X ETD2 propagation path.
+
+Workloads (synthetic model; FP64 throughout):
+  plain   : nuclear cascade, all transported isotopes A <= 56 (19520 states),
+            CMB + two-blackbody EBL, pure Si-28 source, z 3 -> 0, dz 1e-3,
+            one solve per launch sequence (K = 1).
+  batched : the same operator with K = 512 source configurations (spectral
+            index, rigidity cutoff, H/He/N/Si/Fe fractions, (1+z)^m evolution
+            drawn from a seeded generator), one fold and one multi-RHS stage
+            pair per step.
+
+One benchmark "step" is one batched solve (K instances, 3000 ETD2 steps)
+with every input already resident in HBM (the host-side setup happens
+before the timed region).  The JSON line reports batched solves/s as
+``value``, the plain solves/s beside it, the stage-kernel roofline measured
+with HIP events on the launch stream, and the CPU oracle timed on a bounded
+sample of the batched workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = (
+    "solves/sec at A≤56 (batched vs plain, rel err≤1e-8); adjoint/forward ratio"
+)
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3, help="timed batched solves")
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--batch", type=int, default=512)
+    p.add_argument("--plain-solves", type=int, default=3)
+    p.add_argument("--cpu-sample-steps", type=int, default=12)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-events", action="store_true")
+    return p.parse_args()
+
+
+def timed(fn, torch, reps):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def stage_event_times(solver, plan, torch, lib_mod):
+    """Average duration (s) of each stage kernel over one full solve, HIP events."""
+    import ctypes
+
+    lib = lib_mod.load()
+    stream = torch.cuda.current_stream()
+    nsteps = plan["nsteps"]
+    plan["x"].zero_()
+    solver.fold_windows(plan)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * nsteps + 1)]
+    ev[0].record(stream)
+    for k in range(nsteps):
+        for s in (1, 2):
+            lib_mod.check(
+                lib.prh_etd2_stage(ctypes.byref(plan["op"]), ctypes.byref(plan["sch"]),
+                                   ctypes.byref(plan["st"]), k, s, plan["stream"]),
+                "stage",
+            )
+            ev[2 * k + s].record(stream)
+    torch.cuda.synchronize()
+    t1 = [ev[2 * k].elapsed_time(ev[2 * k + 1]) for k in range(nsteps)]
+    t2 = [ev[2 * k + 1].elapsed_time(ev[2 * k + 2]) for k in range(nsteps)]
+    return float(np.mean(t1)) * 1e-3, float(np.mean(t2)) * 1e-3
+
+
+def stage_bytes(plan, n, K):
+    """Algorithmic HBM bytes of one stage launch (x/a/fphi streams + matrix + sources)."""
+    lay = plan["layout"]
+    state = 24.0 * n * K
+    matrix = 8.0 * (lay.zslots + n) + 16.0 * lay.blocks.shape[0]
+    src = plan["bufs"]["src_grid"].numel() * 8.0
+    return state + matrix + src
+
+
+def fold_flops(plan):
+    return 2.0 * plan["fold_rows"] * plan["dph"] * plan["nwin"]
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(
+            "bench.py measures one GPU: the ETD2 path shards by instance with no per-step "
+            "exchange, and this build runs and reports the single-GPU share only.",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+    import torch
+
+    from prince_amd import _lib, workloads
+    from prince_amd.solvers.propagation import (
+        MultiRHSPropagationSolverETD2,
+        UHECRPropagationSolverETD2,
+    )
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(1)
+    t0 = time.perf_counter()
+    run = workloads.nuclear_run()
+    n = run.dim_states
+    log(f"model built in {time.perf_counter() - t0:.1f}s (n={n})")
+
+    # ---------------- plain path (K = 1)
+    plain = UHECRPropagationSolverETD2(3.0, 0.0, run)
+    plain.add_source_class(workloads.silicon_source(run))
+    pplan = plain.prepare(1e-3)
+    plain.execute(pplan)
+    torch.cuda.synchronize()
+    t_plain = timed(lambda: plain.execute(pplan), torch, args.plain_solves) / args.plain_solves
+    plain_state = plain._finalize_state(pplan["x"])
+    nsteps = pplan["nsteps"]
+    log(f"plain: {t_plain * 1e3:.2f} ms/solve ({nsteps} steps, {pplan['nwin']} windows)")
+
+    # ---------------- batched path (K = batch)
+    K = args.batch
+    multi = MultiRHSPropagationSolverETD2(3.0, 0.0, run, K=K)
+    srcs = workloads.batch_sources(run, K, seed=0)
+    for k in range(K):
+        multi[k].add_source_class(srcs[k])
+    t0 = time.perf_counter()
+    bplan = multi.prepare(1e-3)
+    log(f"batched plan K={K} prepared in {time.perf_counter() - t0:.1f}s")
+    for _ in range(max(0, args.warmup)):
+        multi.execute(bplan)
+    torch.cuda.synchronize()
+    t_batch = timed(lambda: multi.execute(bplan), torch, args.steps)
+    ms_per = t_batch / args.steps * 1e3
+    batched_sps = K * args.steps / t_batch
+    plain_sps = 1.0 / t_plain
+    log(f"batched: {ms_per:.1f} ms/solve-batch -> {batched_sps:.1f} solves/s")
+
+    # ---------------- parity of the batched columns vs the plain path
+    bstate = multi._finalize_state(bplan["x"])
+    check_cols = [0, K // 3, K - 1]
+    col_dev = []
+    for k in check_cols:
+        one = UHECRPropagationSolverETD2(3.0, 0.0, run)
+        one.add_source_class(srcs[k])
+        one.solve(1e-3)
+        ref = one.state
+        peak = np.abs(ref).max()
+        mask = np.abs(ref) > 1e-30 * peak
+        col_dev.append(float(np.max(np.abs(bstate[mask, k] - ref[mask]) / np.abs(ref[mask]))))
+    log(f"batched-vs-plain column deviation: {max(col_dev):.2e}")
+
+    # ---------------- roofline of the dominant kernel (stage kernels, batched)
+    roof = None
+    kernels = {}
+    if not args.no_events:
+        t1s, t2s = stage_event_times(multi, bplan, torch, _lib)
+        nbytes = stage_bytes(bplan, n, K)
+        avg = 0.5 * (t1s + t2s)
+        achieved = nbytes / avg / 1e9
+        roof = {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "etd2_stage (stage 1 and 2, batched K=%d)" % K,
+            "bytes_per_launch": nbytes,
+            "avg_launch_us": round(avg * 1e6, 2),
+        }
+        kernels["etd2_stage1_us"] = round(t1s * 1e6, 2)
+        kernels["etd2_stage2_us"] = round(t2s * 1e6, 2)
+        # fold GEMM (MFMA) timing
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        reps = 5
+        for _ in range(reps):
+            multi.fold_windows(bplan)
+        ev1.record()
+        torch.cuda.synchronize()
+        tf = ev0.elapsed_time(ev1) * 1e-3 / reps
+        kernels["fold_gemm_us"] = round(tf * 1e6, 1)
+        kernels["fold_gemm_tflops"] = round(fold_flops(bplan) / tf / 1e12, 2)
+        kernels["fold_gemm_frac_fp64_mfma"] = round(fold_flops(bplan) / tf / 78.6e12, 4)
+        log(f"stage1 {t1s*1e6:.1f} us, stage2 {t2s*1e6:.1f} us, {achieved:.0f} GB/s; "
+            f"fold {tf*1e6:.0f} us")
+
+    # ---------------- CPU baseline: oracle on a bounded sample of the batched workload
+    cpu = None
+    if not args.no_cpu:
+        from threadpoolctl import threadpool_limits
+
+        from oracle.etd2_cpu import OracleETD2
+
+        ns = args.cpu_sample_steps
+        zend = 3.0 - ns * 1e-3
+        o = OracleETD2(3.0, zend, run, K=K)
+        for k in range(K):
+            o.add_source_class(srcs[k], k)
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            o.solve(1e-3)
+            tc = time.perf_counter() - t0
+        nsam = len(o.z_grid) - 1
+        per_step = tc / nsam
+        cpu_sps = K / (per_step * nsteps)
+        cpu = {
+            "value": round(cpu_sps, 4),
+            "unit": "solves/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"CPU FP64 oracle, K={K} batched columns, first {nsam} of {nsteps} steps "
+            f"(z 3 -> {zend:.3f}, {len(o.refresh_zs)} rate refreshes) in {tc:.1f}s, "
+            "extrapolated by step count",
+        }
+        log(f"cpu oracle sample: {tc:.1f}s for {nsam} steps -> {cpu_sps:.3f} solves/s")
+
+    out = {
+        "metric": METRIC,
+        "value": round(batched_sps, 2),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"nuclear_batch: A<=56 ({run.spec_man.nspec} species, {n} states), "
+            f"CMB+EBL, K={K} seeded source draws, z 3->0, dz=1e-3",
+            "K": K,
+            "nsteps": nsteps,
+            "windows": pplan["nwin"],
+            "nnz_offdiag": int(pplan["layout"].nnz_off),
+        },
+        "plain_solves_per_s": round(plain_sps, 3),
+        "plain_ms_per_solve": round(t_plain * 1e3, 3),
+        "batched_vs_plain": round(batched_sps / plain_sps, 2),
+        "steps_per_s_batched": round(nsteps * args.steps / t_batch, 1),
+        "steps_per_s_plain": round(nsteps / t_plain, 1),
+        "batched_column_max_rel_dev_vs_plain": max(col_dev),
+        "adjoint_forward_ratio": None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    _ = plain_state
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -5,10 +5,12 @@
  * prh_last_error() then describes the failure (thread-local string).  No
  * C++ exception crosses this boundary.  All pointers to device memory are
  * owned by the caller (the Python host holds them as torch tensors); the
- * library never allocates or frees device memory and keeps no pointer
- * beyond the call, mirroring the caller-owned-buffer rule of the
- * reference's native sparse handle (src/prince_cr/mkl_sparse.py:12-14,
- * 106-108).  Work is enqueued on the given stream and is asynchronous.
+ * library never allocates device memory and keeps no pointer beyond the
+ * call, except inside a captured step graph (prh_graph_*), which holds the
+ * pointers of the operator/schedule/state it was built from until it is
+ * destroyed.  This mirrors the caller-owned-buffer rule of the reference's
+ * native sparse handle (src/prince_cr/mkl_sparse.py:12-14, 106-108).  Work
+ * is enqueued on the given stream and is asynchronous.
  *
  * Data layout in device memory (one GPU, FP64 throughout):
  *   state  x, a, fphi : [nspec][K][d]  (species block, right-hand side,
@@ -21,8 +23,14 @@
  *                       o_lo = j - i, number of offsets nslot, value
  *                       offset voff}; block value (s, i) at voff + s*d + i
  *                       couples row (da, i) to column (mo, i + o_lo + s).
- *   items             : [nitems][6] = {da, blk_begin, blk_end, part,
- *                       nparts, part_base}; one wave per (item, rhs).
+ *   tasks             : [ntasks][6] = {da, blk_begin, blk_end, lead,
+ *                       nparts, kind}; task t runs as wave t % 16 of a
+ *                       workgroup of row group t / 16.  The nparts parts of
+ *                       one species row sit in waves lead .. lead+nparts-1
+ *                       of one workgroup and are summed, in part order,
+ *                       through LDS.  kind: 0 = only diagonal blocks, 1 =
+ *                       only blocks of at most 8 diagonals, 2 = any.
+ *                       da < 0 marks an empty wave.
  */
 #ifndef PRINCE_HIP_H
 #define PRINCE_HIP_H
@@ -33,12 +41,14 @@
 extern "C" {
 #endif
 
+#define PRH_WG_WAVES 16 /* waves per workgroup of the stage kernels */
+
 typedef struct prh_operator {
   int32_t nspec;   /* species blocks */
   int32_t d;       /* energy bins per block (<= 64) */
   int32_t K;       /* right-hand sides */
-  int32_t nitems;  /* work items per stage */
-  const int32_t* items;
+  int32_t ntasks;  /* multiple of PRH_WG_WAVES */
+  const int32_t* tasks;
   const int32_t* blocks;
   int64_t mvals_stride; /* doubles per window in mvals */
   int64_t zslots;       /* offset of the diagonal inside a window */
@@ -52,6 +62,8 @@ typedef struct prh_operator {
   int32_t nsrc;            /* sources per right-hand side (J) */
   int32_t flags;           /* PRH_F_* */
   const double* src_grid;  /* [nslot][J][K][d] injection grids */
+  int32_t kt;              /* right-hand sides per wave: 1, 2, 4 or 8 */
+  int32_t pad_;
 } prh_operator;
 
 #define PRH_F_PARTIAL_DIFF 1 /* continuous-loss stencil term on */
@@ -69,11 +81,10 @@ typedef struct prh_schedule {
 } prh_schedule;
 
 typedef struct prh_state {
-  double* x;        /* [nspec][K][d] solution, updated in place */
-  double* a;        /* [nspec][K][d] stage-1 vector */
-  double* fphi;     /* [nspec][K][d] F(x) of stage 1 */
-  double* partials; /* [nparts_total][K][d] partial row sums */
-  int32_t* counters;/* [nitems][K], zero on entry, left zero on exit */
+  double* x;         /* [nspec][K][d] solution, updated in place */
+  double* a;         /* [nspec][K][d] stage-1 vector */
+  double* fphi;      /* [nspec][K][d] F(x) of stage 1 */
+  int32_t* step_ctl; /* [2] device step window {first, end} used by graphs */
 } prh_state;
 
 /* Library version and last error (thread-local, never NULL). */
@@ -97,12 +108,29 @@ int32_t prh_gather_rows(const double* src, const int64_t* sel, int64_t nrows, in
                         double* dst, void* stream);
 
 /* ETD2 steps k_begin <= k < k_end on the device (two fused stage kernels
- * per step).  Replaces the step loop of src/prince_cr/solvers/etd2.py:478-517
- * with the step body of etd2.py:412-439 / 442-475 and the operator of
+ * per step, launched one by one).  Replaces the step loop of
+ * src/prince_cr/solvers/etd2.py:478-517 with the step body of
+ * etd2.py:412-439 / 442-475 and the operator of
  * src/prince_cr/solvers/propagation.py:1360-1508 (apply_F, diagonal d,
  * kappa = kappa_adia + kappa_pair, injection b). */
 int32_t prh_etd2_steps(const prh_operator* op, const prh_schedule* sch, const prh_state* st,
                        int32_t k_begin, int32_t k_end, void* stream);
+
+/* One stage (1 or 2) of ETD2 step `step`: the same kernels as
+ * prh_etd2_steps, launched individually (per-kernel timing, debugging). */
+int32_t prh_etd2_stage(const prh_operator* op, const prh_schedule* sch, const prh_state* st,
+                       int32_t step, int32_t stage, void* stream);
+
+/* Step graphs: a hipGraph holding `chunk` steps (2*chunk stage kernels and
+ * one step-window advance) whose kernels read their step from
+ * st->step_ctl, so one instantiated graph replays every chunk of a solve.
+ * Replaces the per-window CUDA-graph capture/replay of
+ * src/prince_cr/solvers/propagation.py:2186-2337. */
+typedef struct prh_graph prh_graph;
+int32_t prh_graph_create(const prh_operator* op, const prh_schedule* sch, const prh_state* st,
+                         int32_t chunk, prh_graph** out);
+int32_t prh_graph_run(prh_graph* g, int32_t k_begin, int32_t k_end, void* stream);
+int32_t prh_graph_destroy(prh_graph* g);
 
 #ifdef __cplusplus
 }

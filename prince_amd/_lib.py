@@ -18,7 +18,13 @@ EXPORTED = (
     "prh_fold_gemm",
     "prh_gather_rows",
     "prh_etd2_steps",
+    "prh_etd2_stage",
+    "prh_graph_create",
+    "prh_graph_run",
+    "prh_graph_destroy",
 )
+
+WG_WAVES = 16
 
 PRH_F_PARTIAL_DIFF = 1
 PRH_F_INJECTION = 2
@@ -33,8 +39,8 @@ class PrhOperator(ctypes.Structure):
         ("nspec", _i32),
         ("d", _i32),
         ("K", _i32),
-        ("nitems", _i32),
-        ("items", _p),
+        ("ntasks", _i32),
+        ("tasks", _p),
         ("blocks", _p),
         ("mvals_stride", _i64),
         ("zslots", _i64),
@@ -48,6 +54,8 @@ class PrhOperator(ctypes.Structure):
         ("nsrc", _i32),
         ("flags", _i32),
         ("src_grid", _p),
+        ("kt", _i32),
+        ("pad_", _i32),
     ]
 
 
@@ -69,8 +77,7 @@ class PrhState(ctypes.Structure):
         ("x", _p),
         ("a", _p),
         ("fphi", _p),
-        ("partials", _p),
-        ("counters", _p),
+        ("step_ctl", _p),
     ]
 
 
@@ -107,8 +114,58 @@ def load():
         _i32,
         _p,
     ]
+    lib.prh_etd2_stage.restype = _i32
+    lib.prh_etd2_stage.argtypes = [
+        ctypes.POINTER(PrhOperator),
+        ctypes.POINTER(PrhSchedule),
+        ctypes.POINTER(PrhState),
+        _i32,
+        _i32,
+        _p,
+    ]
+    lib.prh_graph_create.restype = _i32
+    lib.prh_graph_create.argtypes = [
+        ctypes.POINTER(PrhOperator),
+        ctypes.POINTER(PrhSchedule),
+        ctypes.POINTER(PrhState),
+        _i32,
+        ctypes.POINTER(_p),
+    ]
+    lib.prh_graph_run.restype = _i32
+    lib.prh_graph_run.argtypes = [_p, _i32, _i32, _p]
+    lib.prh_graph_destroy.restype = _i32
+    lib.prh_graph_destroy.argtypes = [_p]
     _LIB = lib
     return lib
+
+
+class StepGraph:
+    """Owned handle of a captured step graph (destroyed with the object)."""
+
+    def __init__(self, op, sch, st, chunk):
+        lib = load()
+        h = _p()
+        check(
+            lib.prh_graph_create(ctypes.byref(op), ctypes.byref(sch), ctypes.byref(st), int(chunk),
+                                 ctypes.byref(h)),
+            "step graph capture",
+        )
+        self._h = h
+        self.chunk = int(chunk)
+
+    def run(self, k_begin, k_end, stream):
+        check(load().prh_graph_run(self._h, int(k_begin), int(k_end), stream), "step graph run")
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load().prh_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def check(status, what):

@@ -13,8 +13,8 @@ window, derive:
   of the fold kernel B -- the device gathers those rows once, so every
   later fold writes straight into the split layout (the reference's
   ``build_split_fold``, ``interaction_rates.py:780-822``);
-* the work items of the stage kernels: each species row split into parts
-  of bounded cost, heaviest rows first.
+* the tasks of the stage kernels: each species row cut into parts of
+  bounded cost, the parts of a row packed into one 16-wave workgroup.
 """
 
 from __future__ import annotations
@@ -115,39 +115,73 @@ def block_cost(blocks):
     return 2 + blocks[:, 2].astype(np.int64)
 
 
-def build_items(layout: BlockLayout, part_cost=96):
-    """Work items [da, b_begin, b_end, part, nparts, part_base] and the part count.
+def split_row(cost, b0, b1, nparts):
+    """Cut blocks [b0, b1) into nparts contiguous non-empty ranges of similar cost."""
+    if nparts <= 1 or b1 - b0 <= 1:
+        return [(b0, b1)]
+    c = np.cumsum(cost[b0:b1])
+    total = c[-1]
+    cuts = [b0]
+    for p in range(1, nparts):
+        pos = b0 + int(np.searchsorted(c, total * p / nparts, side="left")) + 1
+        pos = max(pos, cuts[-1] + 1)
+        pos = min(pos, b1 - (nparts - p))
+        cuts.append(pos)
+    cuts.append(b1)
+    return [(cuts[i], cuts[i + 1]) for i in range(nparts)]
 
-    Rows whose total block cost exceeds ``part_cost`` are split into
-    contiguous parts of at most ``part_cost`` (at least one block each);
-    split rows come first so their long waves start early.
+
+PART_DIAG, PART_BAND, PART_GENERIC = 0, 1, 2
+BAND_MAX_SLOTS = 8
+
+
+def part_kind(blocks, b0, b1):
+    """0: only diagonal blocks; 1: only blocks of <= 8 diagonals; 2: otherwise."""
+    if b1 <= b0:
+        return PART_DIAG
+    blk = blocks[b0:b1]
+    if np.all((blk[:, 2] == 1) & (blk[:, 1] == 0)):
+        return PART_DIAG
+    if np.all(blk[:, 2] <= BAND_MAX_SLOTS):
+        return PART_BAND
+    return PART_GENERIC
+
+
+def build_tasks(layout: BlockLayout, part_cost=96, wg_waves=16):
+    """Stage-kernel tasks [da, b_begin, b_end, lead, nparts, kind], wg_waves per workgroup.
+
+    A row whose blocks cost more than ``part_cost`` is cut into up to
+    ``wg_waves`` parts; all parts of a row are packed into consecutive waves
+    of one workgroup (first fit, longest rows first) so they can be summed
+    through LDS.  Unused waves carry da = -1.
     """
     cost = block_cost(layout.blocks)
-    split_items, plain_items = [], []
-    part_base = 0
+    rows = []
     for da in range(layout.nspec):
         b0, b1 = int(layout.brow_ptr[da]), int(layout.brow_ptr[da + 1])
         total = int(cost[b0:b1].sum()) if b1 > b0 else 0
-        if total <= part_cost or b1 - b0 <= 1:
-            plain_items.append([da, b0, b1, 0, 1, 0])
-            continue
-        bounds = [b0]
-        acc = 0
-        for b in range(b0, b1):
-            if acc > 0 and acc + int(cost[b]) > part_cost:
-                bounds.append(b)
-                acc = 0
-            acc += int(cost[b])
-        bounds.append(b1)
-        nparts = len(bounds) - 1
-        if nparts == 1:
-            plain_items.append([da, b0, b1, 0, 1, 0])
-            continue
-        for p in range(nparts):
-            split_items.append([da, bounds[p], bounds[p + 1], p, nparts, part_base])
-        part_base += nparts
-    items = np.asarray(split_items + plain_items, dtype=np.int32).reshape(-1, 6)
-    return items, part_base
+        nparts = max(1, min(wg_waves, -(-total // part_cost), b1 - b0))
+        rows.append((da, split_row(cost, b0, b1, nparts)))
+    rows.sort(key=lambda r: -len(r[1]))
+    wgs = []
+    for da, parts in rows:
+        np_ = len(parts)
+        target = None
+        for g in wgs:
+            if wg_waves - len(g) >= np_:
+                target = g
+                break
+        if target is None:
+            target = []
+            wgs.append(target)
+        lead = len(target)
+        for b0, b1 in parts:
+            target.append([da, b0, b1, lead, np_, part_kind(layout.blocks, b0, b1)])
+    tasks = []
+    for g in wgs:
+        tasks.extend(g)
+        tasks.extend([[-1, 0, 0, 0, 0, 0]] * (wg_waves - len(g)))
+    return np.asarray(tasks, dtype=np.int32).reshape(-1, 6)
 
 
 def fd_tables(block):

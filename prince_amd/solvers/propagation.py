@@ -31,7 +31,7 @@ from .. import _lib
 from ..model.cosmology import H
 from ..model.partial_diff import DifferentialOperator
 from ..model.units import PRINCE_UNITS
-from .layout import build_block_layout, build_items, fd_tables
+from .layout import build_block_layout, build_tasks, fd_tables
 from .result import UHECRPropagationResult
 
 
@@ -211,8 +211,25 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
     first and last step.
     """
 
-    #: parts of at most this many cost units per wave (autotuned when K > 1)
-    part_cost = 96
+    #: steps per captured hipGraph chunk
+    graph_chunk = 64
+
+    def part_cost_for(self, K):
+        """Cost units per row part (a diagonal block costs 3, a band block 2 + nslot)."""
+        tuned = getattr(self, "_tuned", {}).get(("part_cost", K))
+        if tuned is not None:
+            return tuned
+        return 24 if K == 1 else 96
+
+    def kt_for(self, K):
+        """Right-hand sides per wave: the largest of 8, 4, 2, 1 not above K."""
+        tuned = getattr(self, "_tuned", {}).get(("kt", K))
+        if tuned is not None:
+            return tuned
+        for kt in (8, 4, 2, 1):
+            if kt <= K:
+                return kt
+        return 1
 
     def __init__(self, *args, **kwargs):
         super().__init__(*args, **kwargs)
@@ -228,7 +245,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
 
     # ---------------------------------------------------------------- hooks
     def _reset_for_solve(self):
-        self._plan = None
+        self.close()
 
     def _ensure_D_split(self):
         if self._D_tables is None:
@@ -252,6 +269,8 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         return np.ascontiguousarray(host[:, 0] if K == 1 else host)
 
     def close(self):
+        if self._plan is not None and self._plan.get("graph") is not None:
+            self._plan["graph"].close()
         self._plan = None
 
     # ---------------------------------------------------------------- sources
@@ -376,16 +395,8 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         phot_dev = torch.from_numpy(np.ascontiguousarray(phot)).to(dev)
         mvals = torch.empty((nwin, stride), dtype=torch.float64, device=dev)
         t1 = time.perf_counter()
-        _lib.check(
-            self._lib.prh_fold_gemm(
-                lay["B_sel"].data_ptr(), stride, cache.dph, phot_dev.data_ptr(), nwin,
-                mvals.data_ptr(), stride, stream,
-            ),
-            "window fold",
-        )
-        if not self.enable_photohad_losses:
-            mvals.zero_()
-        items, nparts = build_items(layout, self.part_cost)
+        tasks = build_tasks(layout, self.part_cost_for(K), _lib.WG_WAVES)
+        kt = self.kt_for(K)
         col_sources = self._column_sources()
         has_src = any(len(c) for c in col_sources) and self.enable_injection_jacobian
         h, dldz, hadia, finj, coef, J = self._schedule_arrays(z_grid, col_sources)
@@ -403,7 +414,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         if has_src:
             flags |= _lib.PRH_F_INJECTION
         bufs = {
-            "items": dv(items, np.int32),
+            "tasks": dv(tasks, np.int32),
             "blocks": dv(layout.blocks if layout.blocks.size else np.zeros((1, 4)), np.int32),
             "mvals": mvals,
             "egrid": dv(self.prince_run.cr_grid.grid, np.float64),
@@ -421,47 +432,91 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             "src_coef": dv(coef, np.float64),
             "a": torch.zeros((nspec, K, d), dtype=torch.float64, device=dev),
             "fphi": torch.zeros((nspec, K, d), dtype=torch.float64, device=dev),
-            "partials": torch.zeros((max(1, nparts), K, d), dtype=torch.float64, device=dev),
-            "counters": torch.zeros((max(1, nparts), K), dtype=torch.int32, device=dev),
+            "step_ctl": torch.zeros(2, dtype=torch.int32, device=dev),
         }
         op = _lib.PrhOperator(
-            nspec=nspec, d=d, K=K, nitems=items.shape[0],
-            items=bufs["items"].data_ptr(), blocks=bufs["blocks"].data_ptr(),
+            nspec=nspec, d=d, K=K, ntasks=tasks.shape[0],
+            tasks=bufs["tasks"].data_ptr(), blocks=bufs["blocks"].data_ptr(),
             mvals_stride=stride, zslots=layout.zslots, mvals=mvals.data_ptr(),
             egrid=bufs["egrid"].data_ptr(), dcoef=bufs["dcoef"].data_ptr(),
             ddiag=bufs["ddiag"].data_ptr(), pscale=bufs["pscale"].data_ptr(),
             rpair=bufs["rpair"].data_ptr(), src_slot=bufs["src_slot"].data_ptr(),
-            nsrc=J, flags=flags, src_grid=bufs["src_grid"].data_ptr(),
+            nsrc=J, flags=flags, src_grid=bufs["src_grid"].data_ptr(), kt=kt, pad_=0,
         )
         sch = _lib.PrhSchedule(
             nsteps=len(h), pad_=0, h=bufs["h"].data_ptr(), dldz=bufs["dldz"].data_ptr(),
             hadia=bufs["hadia"].data_ptr(), finj=bufs["finj"].data_ptr(),
             win=bufs["win"].data_ptr(), src_coef=bufs["src_coef"].data_ptr(),
         )
+        x = torch.zeros((nspec, K, d), dtype=torch.float64, device=dev)
+        st = _lib.PrhState(
+            x=x.data_ptr(), a=bufs["a"].data_ptr(), fphi=bufs["fphi"].data_ptr(),
+            step_ctl=bufs["step_ctl"].data_ptr(),
+        )
+        bufs["phot"] = phot_dev
+        bufs["B_sel"] = lay["B_sel"]
         self._plan = {
-            "op": op, "sch": sch, "bufs": bufs, "nsteps": len(h), "layout": layout,
-            "nwin": nwin, "zs": zs, "items": items, "nparts": nparts, "stream": stream,
+            "op": op, "sch": sch, "st": st, "x": x, "bufs": bufs, "nsteps": len(h),
+            "layout": layout, "nwin": nwin, "zs": zs, "tasks": tasks, "kt": kt,
+            "stream": stream, "stride": stride, "dph": cache.dph, "K": K,
+            "fold_rows": stride, "graph": None,
         }
+        if getattr(self.backend, "use_hip_graphs", True) and len(h) >= self.graph_chunk:
+            self._plan["graph"] = _lib.StepGraph(op, sch, st, self.graph_chunk)
         self.refresh_zs = zs
         self.timings["setup_host_s"] = t1 - t0
         return self._plan
 
-    def _run_integration(self, state, z_grid, step_hook=None):
-        torch = _torch()
-        plan = self._ensure_apply_F(z_grid)
-        st = _lib.PrhState(
-            x=state.data_ptr(), a=plan["bufs"]["a"].data_ptr(),
-            fphi=plan["bufs"]["fphi"].data_ptr(), partials=plan["bufs"]["partials"].data_ptr(),
-            counters=plan["bufs"]["counters"].data_ptr(),
+    # ---------------------------------------------------------------- device run
+    def prepare(self, dz=1e-3):
+        """Host-side setup of a solve; every input ends up resident in HBM."""
+        z_grid = z_grid_for(self.initial_z, self.final_z, dz)
+        self._reset_for_solve()
+        self._ensure_D_split()
+        return self._ensure_apply_F(z_grid)
+
+    def fold_windows(self, plan):
+        """Rate fold of every refresh window (one MFMA GEMM) into the plan's mvals."""
+        b = plan["bufs"]
+        _lib.check(
+            self._lib.prh_fold_gemm(
+                b["B_sel"].data_ptr(), plan["fold_rows"], plan["dph"], b["phot"].data_ptr(),
+                plan["nwin"], b["mvals"].data_ptr(), plan["stride"], plan["stream"],
+            ),
+            "window fold",
         )
-        plan["state"] = st
+        if not self.enable_photohad_losses:
+            b["mvals"].zero_()
+
+    def execute(self, plan=None, k_begin=0, k_end=None):
+        """Device part of a solve: zero state, fold, ETD2 steps (asynchronous)."""
+        plan = plan if plan is not None else self._plan
+        if k_begin == 0:
+            plan["x"].zero_()
+            self.fold_windows(plan)
+        k_end = plan["nsteps"] if k_end is None else k_end
+        self._launch_steps(plan, k_begin, k_end)
+        return plan["x"]
+
+    def _launch_steps(self, plan, k_begin, k_end):
+        if plan["graph"] is not None:
+            plan["graph"].run(k_begin, k_end, plan["stream"])
+            return
         _lib.check(
             self._lib.prh_etd2_steps(
-                ctypes.byref(plan["op"]), ctypes.byref(plan["sch"]), ctypes.byref(st), 0,
-                plan["nsteps"], plan["stream"],
+                ctypes.byref(plan["op"]), ctypes.byref(plan["sch"]), ctypes.byref(plan["st"]),
+                k_begin, k_end, plan["stream"],
             ),
             "ETD2 steps",
         )
+
+    def _run_integration(self, state, z_grid, step_hook=None):
+        torch = _torch()
+        plan = self._ensure_apply_F(z_grid)
+        plan["x"].copy_(state)
+        self.fold_windows(plan)
+        self._launch_steps(plan, 0, plan["nsteps"])
+        state.copy_(plan["x"])
         torch.cuda.synchronize(self.device)
         return state
 

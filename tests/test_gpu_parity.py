@@ -80,3 +80,24 @@ def test_multi_rhs_matches_columns():
         dev, nb = rel_dev(g[k].state, r.state[:, k])
         print(f"column {k}: {nb} bins, max rel dev {dev:.3e}")
         assert dev <= TOL
+
+
+def test_kt_and_graph_variants_agree():
+    """Every right-hand-side tiling and the graph replay give the same state."""
+    run = workloads.nuclear_run()
+    K = 8
+    srcs = workloads.batch_sources(run, K, seed=2)
+    states = {}
+    for kt in (1, 2, 4, 8):
+        for graphs in (True, False):
+            g = MultiRHSPropagationSolverETD2(0.15, 0.0, run, K=K)
+            g._tuned = {("kt", K): kt}
+            g.backend.use_hip_graphs = graphs
+            for k in range(K):
+                g[k].add_source_class(srcs[k])
+            g.solve(dz=1e-3)
+            states[(kt, graphs)] = g.state.copy()
+    run.backend.use_hip_graphs = True
+    base = states[(8, True)]
+    for key, st in states.items():
+        assert np.array_equal(st, base), key

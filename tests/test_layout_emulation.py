@@ -13,7 +13,7 @@ from prince_amd import workloads
 from prince_amd.model.cosmology import H
 from prince_amd.model.partial_diff import single_block
 from prince_amd.model.units import PRINCE_UNITS
-from prince_amd.solvers.layout import build_block_layout, build_items, fd_tables
+from prince_amd.solvers.layout import build_block_layout, build_tasks, fd_tables
 from prince_amd.solvers.propagation import refresh_schedule, z_grid_for
 
 from .parity import rel_dev
@@ -31,10 +31,22 @@ def emulate(run, src, zi, zf, dz=1e-3, part_cost=24):
     B = ir.batch_matrix
     m0 = B @ phot[0]
     lay = build_block_layout(ir.coupling_mat.indptr, ir.coupling_mat.indices, m0, nspec, d)
-    items, nparts = build_items(lay, part_cost)
+    tasks = build_tasks(lay, part_cost)
     assert lay.blocks.shape[0] == 0 or int((lay.blocks[:, 3] + lay.blocks[:, 2] * d).max()) <= lay.zslots
-    assert items[:, 0].max() < nspec
-    assert set(items[:, 0]) == set(range(nspec))
+    assert tasks.shape[0] % 16 == 0
+    assert tasks[:, 0].max() < nspec
+    assert set(tasks[tasks[:, 0] >= 0, 0]) == set(range(nspec))
+    # every row's parts are consecutive waves of one workgroup, in block order
+    for t, (da, b0, b1, lead, nparts, kind) in enumerate(tasks):
+        if da < 0:
+            continue
+        wg, wv = divmod(t, 16)
+        assert lead <= wv < lead + nparts <= 16
+        first = wg * 16 + lead
+        assert tasks[first, 1] == lay.brow_ptr[da]
+        assert tasks[first + nparts - 1, 2] == lay.brow_ptr[da + 1]
+        if wv > lead:
+            assert tasks[t - 1, 2] == b0
     sel = lay.sel
     Bsel = np.where(sel[:, None] >= 0, B[np.maximum(sel, 0)], 0.0)
     mvals = phot @ Bsel.T  # [nwin, zslots + n]
@@ -65,8 +77,10 @@ def emulate(run, src, zi, zf, dz=1e-3, part_cost=24):
         for stage2 in (False, True):
             xin = a if stage2 else x
             acc_rows = np.zeros((nspec, d))
-            for it in items:
+            for it in tasks:
                 da, b0, b1 = it[0], it[1], it[2]
+                if da < 0:
+                    continue
                 acc = np.zeros(d)
                 for b in range(b0, b1):
                     mo, olo, nslot, voff = lay.blocks[b]
