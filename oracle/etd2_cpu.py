@@ -83,6 +83,27 @@ def phi_factors(h, d):
     return eD, phi1, phi2
 
 
+def etd2_step(state, h, d, apply_F):
+    """One ETD2 step with operator frozen at the step start (etd2.py:412-439).
+
+    F(x) = L_off x + b;  a = e^{hd} x + h phi1(hd) F(x);
+    x <- a + h phi2(hd) (F(a) - F(x)).
+    """
+    eD, phi1, phi2 = phi_factors(h, d)
+    if state.ndim == 2:
+        eD, phi1, phi2 = eD[:, None], phi1[:, None], phi2[:, None]
+    F_phi = apply_F(state)
+    a = eD * state
+    scratch = phi1 * F_phi
+    scratch = scratch * h
+    a = a + scratch
+    F_a = apply_F(a)
+    scratch = F_a - F_phi
+    scratch = scratch * h
+    scratch = scratch * phi2
+    return a + scratch
+
+
 def split_operator(L):
     """(diag(L), L - diag) with zeros eliminated (etd2.py:234-268)."""
     d = np.asarray(L.diagonal(), dtype=np.float64)
@@ -250,19 +271,7 @@ class OracleETD2:
             z0, z1 = z_grid[k], z_grid[k + 1]
             h = z1 - z0
             d, apply_F = self._operator_at(z0)
-            eD, phi1, phi2 = phi_factors(h, d)
-            if state.ndim == 2:
-                eD, phi1, phi2 = eD[:, None], phi1[:, None], phi2[:, None]
-            F_phi = apply_F(state)
-            a = eD * state
-            scratch = phi1 * F_phi
-            scratch = scratch * h
-            a = a + scratch
-            F_a = apply_F(a)
-            scratch = F_a - F_phi
-            scratch = scratch * h
-            scratch = scratch * phi2
-            state = a + scratch
+            state = etd2_step(state, h, d, apply_F)
             if step_callback is not None:
                 step_callback(k, state)
         self.state = state

@@ -1,0 +1,63 @@
+"""C ABI: the in-tree library loads and exports every function the header declares;
+host-side guards of the HIP backend (no GPU needed)."""
+
+import os
+import re
+
+import pytest
+
+from prince_amd import _lib
+from prince_amd.model.config import BackendConfig
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(inc, fn)).read()
+            names |= set(re.findall(r"^\s*(?:int32_t|const char\*)\s+(prh_\w+)\s*\(", txt, re.M))
+    return names
+
+
+def test_library_exports_declared_symbols():
+    lib = _lib.load()
+    declared = _declared()
+    assert declared, "no declarations found"
+    assert declared == set(_lib.EXPORTED)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.prh_version() >= 2
+    assert isinstance(lib.prh_last_error(), bytes)
+
+
+def test_abi_rejects_bad_arguments_without_touching_the_gpu():
+    lib = _lib.load()
+    assert lib.prh_fold_gemm(None, 10, 200, None, 1, None, 10, None) != 0
+    assert b"dph" in lib.prh_last_error()
+    assert lib.prh_gather_rows(None, None, -1, 72, None, None) != 0
+    assert lib.prh_etd2_steps(None, None, None, 0, 1, None) != 0
+    assert b"null" in lib.prh_last_error()
+    assert lib.prh_graph_run(None, 0, 1, None) != 0
+    assert lib.prh_graph_destroy(None) == 0
+
+
+def test_backend_dispatch_and_gpu_requirement():
+    import torch
+
+    from prince_amd import workloads
+    from prince_amd.solvers.propagation import UHECRPropagationSolverETD2
+
+    run = workloads.proton_run()
+    saved = run.backend
+    try:
+        run.backend = BackendConfig(linear_algebra_backend="scipy")
+        with pytest.raises(NotImplementedError):
+            UHECRPropagationSolverETD2(1.0, 0.0, run)
+    finally:
+        run.backend = saved
+    if not torch.cuda.is_available():
+        with pytest.raises(RuntimeError, match="GPU"):
+            UHECRPropagationSolverETD2(1.0, 0.0, run)
