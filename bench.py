@@ -51,6 +51,8 @@ def parse():
     p.add_argument("--cpu-sample-steps", type=int, default=12)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-events", action="store_true")
+    p.add_argument("--no-adjoint", action="store_true")
+    p.add_argument("--adjoint-batch", type=int, default=64)
     return p.parse_args()
 
 
@@ -92,13 +94,81 @@ def stage_bytes(plan, n, K):
     """Algorithmic HBM bytes of one stage launch (x/a/fphi streams + matrix + sources)."""
     lay = plan["layout"]
     state = 24.0 * n * K
-    matrix = 8.0 * (lay.zslots + n) + 16.0 * lay.blocks.shape[0]
+    matrix = 8.0 * (lay.fwd.zslots + n) + 16.0 * lay.blocks.shape[0]
     src = plan["bufs"]["src_grid"].numel() * 8.0
     return state + matrix + src
 
 
 def fold_flops(plan):
     return 2.0 * plan["fold_rows"] * plan["dph"] * plan["nwin"]
+
+
+def adjoint_leg(K, torch, log):
+    """Adjoint/forward cost on the nuclear run, K instances, z 3 -> 0 (device time).
+
+    forward  = rate fold + 3000 ETD2 steps keeping a checkpoint every 64 steps;
+    backward = loss gradient w.r.t. the final state + recompute of every chunk +
+               the adjoint sweep (two transposed stage kernels per step).
+    A directional central difference (relative step 1e-6) checks the gradient.
+    """
+    from prince_amd.adjoint import AdjointProblem, draw_params
+
+    prob = AdjointProblem(K=K, z_max=3.0)
+    theta = draw_params(K, seed=7)
+    prob.target()
+    solver = prob._solver(theta)
+    plan = solver.prepare(1e-3, with_adjoint=True)
+
+    def fwd():
+        solver.forward_checkpointed(plan)
+
+    fwd()
+    torch.cuda.synchronize()
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fwd()
+    torch.cuda.synchronize()
+    t_f = (time.perf_counter() - t0) / reps
+    st = plan["x"].permute(0, 2, 1)
+
+    def bwd():
+        stv = st.detach().clone().requires_grad_(True)
+        losses, lp = prob.loss_of_state(stv)
+        losses.sum().backward()
+        ab = solver.adjoint_sweep(plan, stv.grad.permute(0, 2, 1).contiguous())
+        return losses, lp, ab
+
+    bwd()
+    fwd()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses, lp, ab = bwd()
+    torch.cuda.synchronize()
+    t_b = time.perf_counter() - t0
+    grad = prob.contract(solver, plan, ab, theta)
+    # directional check: d/de L(theta + e v) vs grad . v
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(theta.shape)
+    scale = np.maximum(np.abs(theta), 1.0)
+    scale[:, 1] = 1.0 / np.log(10.0)
+    h = 1e-6
+    lpl = prob.loss_of_state(prob.forward(theta + h * v * scale, with_adjoint=False)[2], *lp)[0]
+    lmi = prob.loss_of_state(prob.forward(theta - h * v * scale, with_adjoint=False)[2], *lp)[0]
+    fd = ((lpl - lmi) / (2 * h)).detach().cpu().numpy()
+    an = (grad * v * scale).sum(axis=1)
+    rel = float(np.max(np.abs(an - fd) / np.maximum(np.abs(fd), 1e-300)))
+    log(f"adjoint K={K}: forward {t_f*1e3:.1f} ms, backward {t_b*1e3:.1f} ms, "
+        f"ratio {t_b/t_f:.2f}, directional FD rel dev {rel:.2e}")
+    return {
+        "K": K,
+        "forward_ms": round(t_f * 1e3, 2),
+        "backward_ms": round(t_b * 1e3, 2),
+        "ratio": round(t_b / t_f, 3),
+        "fd_directional_max_rel_dev": rel,
+        "params_per_instance": 8,
+        "checkpoint_every": 64,
+    }
 
 
 def main():
@@ -205,6 +275,11 @@ def main():
         log(f"stage1 {t1s*1e6:.1f} us, stage2 {t2s*1e6:.1f} us, {achieved:.0f} GB/s; "
             f"fold {tf*1e6:.0f} us")
 
+    # ---------------- adjoint (C4 geometry): backward / forward device time
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_leg(args.adjoint_batch, torch, log)
+
     # ---------------- CPU baseline: oracle on a bounded sample of the batched workload
     cpu = None
     if not args.no_cpu:
@@ -262,7 +337,8 @@ def main():
         "steps_per_s_batched": round(nsteps * args.steps / t_batch, 1),
         "steps_per_s_plain": round(nsteps / t_plain, 1),
         "batched_column_max_rel_dev_vs_plain": max(col_dev),
-        "adjoint_forward_ratio": None,
+        "adjoint_forward_ratio": adjoint["ratio"] if adjoint else None,
+        "adjoint": adjoint,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,

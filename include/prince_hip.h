@@ -17,8 +17,12 @@
  *                       energy bin); for K = 1 this is the reference
  *                       state vector order princeidx*d + i.
  *   mvals             : [nwin][mvals_stride]; per rate-cache window the
- *                       off-diagonal coupling values in block-band layout
- *                       (zslots doubles) followed by the diagonal (n).
+ *                       diagonal of M (n doubles at diag_off) and the
+ *                       off-diagonal values in block-band layout (at the
+ *                       block offsets voff).  With two photon-field
+ *                       components, mvals_e holds the second component's
+ *                       fold and row k of the operator uses
+ *                       mvals + escale[k] * mvals_e.
  *   blocks            : [nblocks][4] = {mother species, first offset
  *                       o_lo = j - i, number of offsets nslot, value
  *                       offset voff}; block value (s, i) at voff + s*d + i
@@ -51,7 +55,7 @@ typedef struct prh_operator {
   const int32_t* tasks;
   const int32_t* blocks;
   int64_t mvals_stride; /* doubles per window in mvals */
-  int64_t zslots;       /* offset of the diagonal inside a window */
+  int64_t diag_off;     /* offset of the diagonal inside a window */
   const double* mvals;
   const double* egrid;  /* [d] bin-centre energies (adiabatic loss) */
   const double* dcoef;  /* [d][7] FD coefficients for offsets -3..3 */
@@ -64,6 +68,9 @@ typedef struct prh_operator {
   const double* src_grid;  /* [nslot][J][K][d] injection grids */
   int32_t kt;              /* right-hand sides per wave: 1, 2, 4 or 8 */
   int32_t pad_;
+  const double* mvals_e;   /* [nwin][mvals_stride] second component or NULL */
+  const double* rpair_e;   /* [nwin][d] second component's pair rate or NULL */
+  const double* escale;    /* [K] second component's scale per rhs or NULL */
 } prh_operator;
 
 #define PRH_F_PARTIAL_DIFF 1 /* continuous-loss stencil term on */
@@ -85,7 +92,33 @@ typedef struct prh_state {
   double* a;         /* [nspec][K][d] stage-1 vector */
   double* fphi;      /* [nspec][K][d] F(x) of stage 1 */
   int32_t* step_ctl; /* [2] device step window {first, end} used by graphs */
+  /* Optional record of each step of a chunk (NULL: off): x, a, F(x), F(a)
+   * at [step - rec_first][nspec][K][d] (graph replay: [step - step_ctl[0]]). */
+  double* rec_x;
+  double* rec_a;
+  double* rec_fx;
+  double* rec_fa;
+  int32_t rec_first;
+  int32_t rec_count;
 } prh_state;
+
+/* Adjoint sweep buffers (all [nspec][K][d] unless noted). */
+typedef struct prh_adjoint {
+  double* lam;   /* dJ/dx at the end of the step on entry, at its start on exit */
+  double* lam_a; /* dJ/da of the stage-1 vector */
+  double* u;     /* h phi2 lam' (written by the previous adjoint step or the seed) */
+  double* v;     /* h phi1 lam_a - u */
+  const double* rec_x; /* recorded forward chunk, see prh_state */
+  const double* rec_a;
+  const double* rec_fx;
+  const double* rec_fa;
+  int32_t rec_first;
+  int32_t rec_count;
+  double* gsrc0;       /* [nslot][J][K][d] += dl/dz*cm2sec * coef * h phi1 lam_a, or NULL */
+  double* gsrc1;       /* same weighted with ln(1+z) */
+  const double* lnz1;  /* [nsteps] ln(1 + z_step) */
+  double* gdot;        /* += per-element terms of dJ/d(escale), or NULL */
+} prh_adjoint;
 
 /* Library version and last error (thread-local, never NULL). */
 int32_t prh_version(void);
@@ -120,6 +153,19 @@ int32_t prh_etd2_steps(const prh_operator* op, const prh_schedule* sch, const pr
  * prh_etd2_steps, launched individually (per-kernel timing, debugging). */
 int32_t prh_etd2_stage(const prh_operator* op, const prh_schedule* sch, const prh_state* st,
                        int32_t step, int32_t stage, void* stream);
+
+/* Adjoint of ETD2 step `step` (two transposed stage kernels).  opT is the
+ * operator in transposed block layout with the transposed FD stencil.  No
+ * counterpart in the reference (which has no adjoint); it is the reverse of
+ * prh_etd2_steps for one step. */
+int32_t prh_adjoint_step(const prh_operator* opT, const prh_schedule* sch, const prh_adjoint* adj,
+                         int32_t step, void* stream);
+/* Adjoint steps k_end-1 down to k_begin (prh_adjoint_step in a loop). */
+int32_t prh_adjoint_steps(const prh_operator* opT, const prh_schedule* sch, const prh_adjoint* adj,
+                          int32_t k_begin, int32_t k_end, void* stream);
+/* u = h phi2 lam for step `step`, before the first prh_adjoint_step of a sweep. */
+int32_t prh_adjoint_seed(const prh_operator* opT, const prh_schedule* sch, const prh_adjoint* adj,
+                         int32_t step, void* stream);
 
 /* Step graphs: a hipGraph holding `chunk` steps (2*chunk stage kernels and
  * one step-window advance) whose kernels read their step from

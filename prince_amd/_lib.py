@@ -22,6 +22,9 @@ EXPORTED = (
     "prh_graph_create",
     "prh_graph_run",
     "prh_graph_destroy",
+    "prh_adjoint_step",
+    "prh_adjoint_seed",
+    "prh_adjoint_steps",
 )
 
 WG_WAVES = 16
@@ -43,7 +46,7 @@ class PrhOperator(ctypes.Structure):
         ("tasks", _p),
         ("blocks", _p),
         ("mvals_stride", _i64),
-        ("zslots", _i64),
+        ("diag_off", _i64),
         ("mvals", _p),
         ("egrid", _p),
         ("dcoef", _p),
@@ -56,6 +59,9 @@ class PrhOperator(ctypes.Structure):
         ("src_grid", _p),
         ("kt", _i32),
         ("pad_", _i32),
+        ("mvals_e", _p),
+        ("rpair_e", _p),
+        ("escale", _p),
     ]
 
 
@@ -78,6 +84,31 @@ class PrhState(ctypes.Structure):
         ("a", _p),
         ("fphi", _p),
         ("step_ctl", _p),
+        ("rec_x", _p),
+        ("rec_a", _p),
+        ("rec_fx", _p),
+        ("rec_fa", _p),
+        ("rec_first", _i32),
+        ("rec_count", _i32),
+    ]
+
+
+class PrhAdjoint(ctypes.Structure):
+    _fields_ = [
+        ("lam", _p),
+        ("lam_a", _p),
+        ("u", _p),
+        ("v", _p),
+        ("rec_x", _p),
+        ("rec_a", _p),
+        ("rec_fx", _p),
+        ("rec_fa", _p),
+        ("rec_first", _i32),
+        ("rec_count", _i32),
+        ("gsrc0", _p),
+        ("gsrc1", _p),
+        ("lnz1", _p),
+        ("gdot", _p),
     ]
 
 
@@ -119,6 +150,25 @@ def load():
         ctypes.POINTER(PrhOperator),
         ctypes.POINTER(PrhSchedule),
         ctypes.POINTER(PrhState),
+        _i32,
+        _i32,
+        _p,
+    ]
+    for name in ("prh_adjoint_step", "prh_adjoint_seed"):
+        fn = getattr(lib, name)
+        fn.restype = _i32
+        fn.argtypes = [
+            ctypes.POINTER(PrhOperator),
+            ctypes.POINTER(PrhSchedule),
+            ctypes.POINTER(PrhAdjoint),
+            _i32,
+            _p,
+        ]
+    lib.prh_adjoint_steps.restype = _i32
+    lib.prh_adjoint_steps.argtypes = [
+        ctypes.POINTER(PrhOperator),
+        ctypes.POINTER(PrhSchedule),
+        ctypes.POINTER(PrhAdjoint),
         _i32,
         _i32,
         _p,

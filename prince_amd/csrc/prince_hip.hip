@@ -125,96 +125,67 @@ __device__ __forceinline__ double shfl_or_zero(double v, int src, int d) {
   return (src >= 0 && src < d) ? r : 0.0;
 }
 
-// step_local: step index (use_ctl = 0) or offset from step_ctl[0] (use_ctl = 1,
-// graph replay; steps at or beyond step_ctl[1] do nothing).
 // Blocks whose loads are issued together before any of them is used: runs
-// of diagonal blocks and of band blocks with at most NSMAX diagonals.
+// of diagonal blocks and of band blocks with at most NSMAX diagonals.  With
+// two photon-field components every block has two value sets.
 constexpr int NSMAX = 8;
-template <int KT>
+template <int KT, bool TWO>
 struct Batch {
-  static constexpr int DIAG = KT <= 2 ? 8 : (KT == 4 ? 4 : 2);
-  static constexpr int BAND = KT == 1 ? 4 : (KT == 2 ? 2 : 1);
+  static constexpr int DIAG = TWO ? (KT <= 2 ? 4 : 2) : (KT <= 2 ? 8 : (KT == 4 ? 4 : 2));
+  static constexpr int BAND = TWO ? (KT <= 2 ? 1 : 1) : (KT == 1 ? 4 : (KT == 2 ? 2 : 1));
 };
 
 enum PartKind { PART_DIAG = 0, PART_BAND = 1, PART_GENERIC = 2 };
 
-// step_local: step index (use_ctl = 0) or offset from step_ctl[0] (use_ctl = 1,
-// graph replay; steps at or beyond step_ctl[1] do nothing).
-template <bool STAGE2, int KT>
-__global__ void __launch_bounds__(STHREADS) etd2_stage(prh_operator op, prh_schedule sch,
-                                                       prh_state st, int step_local,
-                                                       int use_ctl) {
-  __shared__ double red[SWG][KT][WAVE];
-  int step = step_local;
-  if (use_ctl) {
-    step = st.step_ctl[0] + step_local;
-    if (step >= st.step_ctl[1]) return;  // uniform over the grid
-  }
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = threadIdx.x / WAVE;
+// Coupling product of one row part over its blocks, in column order:
+//   accC[kk] = sum_b mC_b (shifted) y_b,   accE[kk] = sum_b mE_b (shifted) y_b
+// for the KT right-hand sides k0 .. k0+KT-1 (accE only with TWO).
+template <int KT, bool TWO>
+__device__ __forceinline__ void couple_part(const prh_operator& op, const double* __restrict__ y,
+                                            const double* __restrict__ mvC,
+                                            const double* __restrict__ mvE, int b_begin,
+                                            int b_end, int kind, int k0, int lane, bool active,
+                                            double (&accC)[KT], double (&accE)[KT]) {
   const int K = op.K;
   const int d = op.d;
-  // XCD-aware placement: workgroup L runs on XCD L % 8; when the number of
-  // right-hand-side groups is a multiple of 8, every XCD owns whole groups
-  // so their state panels stay in that XCD's L2.
-  const int nx = op.ntasks / SWG;
-  const int ny = (K + KT - 1) / KT;
-  const int L = blockIdx.x;
-  int gx, gy;
-  if (ny % 8 == 0) {
-    const int j = L >> 3;
-    gy = (L & 7) + 8 * (j / nx);
-    gx = j % nx;
-  } else {
-    gx = L % nx;
-    gy = L / nx;
-  }
-  const int k0 = gy * KT;
-  const int* tk = op.tasks + 6 * ((int64_t)gx * SWG + wv);
-  const int da = tk[0];
-  const int b_begin = tk[1];
-  const int b_end = tk[2];
-  const int lead = tk[3];
-  const int nparts = tk[4];
-  const int kind = tk[5];
-  const bool active = lane < d;
-
-  const double* __restrict__ xin = STAGE2 ? st.a : st.x;
-  const int w = sch.win[step];
-  const double* __restrict__ mv = op.mvals + (int64_t)w * op.mvals_stride;
-
-  // ---- off-diagonal coupling over this part's blocks, in column order
-  double acc[KT];
 #pragma unroll
-  for (int kk = 0; kk < KT; ++kk) acc[kk] = 0.0;
-  if (da >= 0 && kind == PART_DIAG) {
-    constexpr int UB = Batch<KT>::DIAG;
+  for (int kk = 0; kk < KT; ++kk) {
+    accC[kk] = 0.0;
+    accE[kk] = 0.0;
+  }
+  if (kind == PART_DIAG) {
+    constexpr int UB = Batch<KT, TWO>::DIAG;
     for (int b = b_begin; b < b_end; b += UB) {
-      double m[UB];
+      double mc[UB], me[UB];
       double xv[UB][KT];
 #pragma unroll
       for (int u = 0; u < UB; ++u) {
-        const int bb = (b + u < b_end) ? b + u : b_end - 1;
+        const bool valid = b + u < b_end;
+        const int bb = valid ? b + u : b_end - 1;
         const int* blk = op.blocks + 4 * bb;
         const int mo = blk[0];
         const int64_t voff = blk[3];
-        m[u] = (active && b + u < b_end) ? mv[voff + lane] : 0.0;
+        mc[u] = (active && valid) ? mvC[voff + lane] : 0.0;
+        me[u] = (TWO && active && valid) ? mvE[voff + lane] : 0.0;
 #pragma unroll
         for (int kk = 0; kk < KT; ++kk) {
           const int k = k0 + kk;
-          xv[u][kk] = (active && k < K) ? xin[((int64_t)mo * K + k) * d + lane] : 0.0;
+          xv[u][kk] = (active && k < K) ? y[((int64_t)mo * K + k) * d + lane] : 0.0;
         }
       }
 #pragma unroll
       for (int u = 0; u < UB; ++u) {
 #pragma unroll
-        for (int kk = 0; kk < KT; ++kk) acc[kk] += m[u] * xv[u][kk];
+        for (int kk = 0; kk < KT; ++kk) {
+          accC[kk] += mc[u] * xv[u][kk];
+          if (TWO) accE[kk] += me[u] * xv[u][kk];
+        }
       }
     }
-  } else if (da >= 0 && kind == PART_BAND) {
-    constexpr int UB = Batch<KT>::BAND;
+  } else if (kind == PART_BAND) {
+    constexpr int UB = Batch<KT, TWO>::BAND;
     for (int b = b_begin; b < b_end; b += UB) {
-      double m[UB][NSMAX];
+      double mc[UB][NSMAX], me[UB][TWO ? NSMAX : 1];
       double xv[UB][KT];
       int olo[UB], ns[UB];
 #pragma unroll
@@ -230,12 +201,13 @@ __global__ void __launch_bounds__(STHREADS) etd2_stage(prh_operator op, prh_sche
 #pragma unroll
         for (int s2 = 0; s2 < NSMAX; ++s2) {
           const int sc = s2 < nsl ? s2 : nsl - 1;
-          m[u][s2] = active ? mv[voff + (int64_t)sc * d + lane] : 0.0;
+          mc[u][s2] = active ? mvC[voff + (int64_t)sc * d + lane] : 0.0;
+          if (TWO) me[u][TWO ? s2 : 0] = active ? mvE[voff + (int64_t)sc * d + lane] : 0.0;
         }
 #pragma unroll
         for (int kk = 0; kk < KT; ++kk) {
           const int k = k0 + kk;
-          xv[u][kk] = (active && k < K) ? xin[((int64_t)mo * K + k) * d + lane] : 0.0;
+          xv[u][kk] = (active && k < K) ? y[((int64_t)mo * K + k) * d + lane] : 0.0;
         }
       }
 #pragma unroll
@@ -245,12 +217,16 @@ __global__ void __launch_bounds__(STHREADS) etd2_stage(prh_operator op, prh_sche
           if (s2 < ns[u]) {
             const int src = lane + olo[u] + s2;
 #pragma unroll
-            for (int kk = 0; kk < KT; ++kk) acc[kk] += m[u][s2] * shfl_or_zero(xv[u][kk], src, d);
+            for (int kk = 0; kk < KT; ++kk) {
+              const double xs = shfl_or_zero(xv[u][kk], src, d);
+              accC[kk] += mc[u][s2] * xs;
+              if (TWO) accE[kk] += me[u][TWO ? s2 : 0] * xs;
+            }
           }
         }
       }
     }
-  } else if (da >= 0) {
+  } else {
     for (int b = b_begin; b < b_end; ++b) {
       const int* blk = op.blocks + 4 * b;
       const int mo = blk[0];
@@ -261,25 +237,34 @@ __global__ void __launch_bounds__(STHREADS) etd2_stage(prh_operator op, prh_sche
 #pragma unroll
       for (int kk = 0; kk < KT; ++kk) {
         const int k = k0 + kk;
-        xv[kk] = (active && k < K) ? xin[((int64_t)mo * K + k) * d + lane] : 0.0;
+        xv[kk] = (active && k < K) ? y[((int64_t)mo * K + k) * d + lane] : 0.0;
       }
       for (int s2 = 0; s2 < nslot; ++s2) {
         const int src = lane + olo + s2;
-        const double m = active ? mv[voff + (int64_t)s2 * d + lane] : 0.0;
+        const double mC = active ? mvC[voff + (int64_t)s2 * d + lane] : 0.0;
+        const double mE = (TWO && active) ? mvE[voff + (int64_t)s2 * d + lane] : 0.0;
 #pragma unroll
-        for (int kk = 0; kk < KT; ++kk) acc[kk] += m * shfl_or_zero(xv[kk], src, d);
+        for (int kk = 0; kk < KT; ++kk) {
+          const double xs = shfl_or_zero(xv[kk], src, d);
+          accC[kk] += mC * xs;
+          if (TWO) accE[kk] += mE * xs;
+        }
       }
     }
   }
+}
 
-  // ---- combine the parts of split rows through LDS (in part order)
+// Sum the parts of split rows through LDS, in part order.  Every wave of the
+// workgroup must call this (it contains the workgroup barrier).
+template <int KT>
+__device__ __forceinline__ void reduce_parts(double (*red)[KT][WAVE], int wv, int lane, int lead,
+                                             int nparts, double (&acc)[KT]) {
   if (nparts > 1) {
 #pragma unroll
     for (int kk = 0; kk < KT; ++kk) red[wv][kk][lane] = acc[kk];
   }
   __syncthreads();
-  if (da < 0 || wv != lead) return;
-  if (nparts > 1) {
+  if (nparts > 1 && wv == lead) {
 #pragma unroll
     for (int kk = 0; kk < KT; ++kk) {
       double sum = red[lead][kk][lane];
@@ -287,51 +272,140 @@ __global__ void __launch_bounds__(STHREADS) etd2_stage(prh_operator op, prh_sche
       acc[kk] = sum;
     }
   }
+}
+
+// workgroup -> (row group, right-hand-side group), XCD-aware when the number
+// of right-hand-side groups is a multiple of 8 (workgroup L runs on XCD L % 8).
+__device__ __forceinline__ void wg_coords(int ntasks, int K, int kt, int& gx, int& gy) {
+  const int nx = ntasks / SWG;
+  const int ny = (K + kt - 1) / kt;
+  const int L = blockIdx.x;
+  if (ny % 8 == 0) {
+    const int j = L >> 3;
+    gy = (L & 7) + 8 * (j / nx);
+    gx = j % nx;
+  } else {
+    gx = L % nx;
+    gy = L / nx;
+  }
+}
+
+// diagonal phi factors (solvers/etd2.py:329-390)
+__device__ __forceinline__ double phi1_of(double hd, double e) {
+  return (fabs(hd) > PHI1_SMALL) ? (e - 1.0) / hd : (hd * (1.0 / 6.0) + 0.5) * hd + 1.0;
+}
+__device__ __forceinline__ double phi2_of(double hd, double e) {
+  return (fabs(hd) > PHI2_SMALL) ? ((e - 1.0) - hd) / (hd * hd)
+                                 : (hd * (1.0 / 24.0) + (1.0 / 6.0)) * hd + 0.5;
+}
+
+// Row-local coefficients of step `step` for row da, rhs scale s:
+// kappa, kappa's EBL part, and the diagonal d (propagation.py:1389-1422).
+struct RowCoef {
+  double kap, kapE, dval, dE;
+};
+template <bool TWO>
+__device__ __forceinline__ RowCoef row_coef(const prh_operator& op, const prh_schedule& sch,
+                                            int step, int w, int da, int lane, double s,
+                                            bool pdiff) {
+  RowCoef r;
+  const int d = op.d;
+  const double dl = sch.dldz[step];
+  const double E = op.egrid[lane];
+  const double rc = op.rpair[(int64_t)w * d + lane];
+  const double re = TWO ? op.rpair_e[(int64_t)w * d + lane] : 0.0;
+  const double* mv = op.mvals + (int64_t)w * op.mvals_stride;
+  const double mdc = mv[op.diag_off + (int64_t)da * d + lane];
+  const double mde = TWO ? op.mvals_e[(int64_t)w * op.mvals_stride + op.diag_off +
+                                      (int64_t)da * d + lane]
+                         : 0.0;
+  r.kap = sch.hadia[step] * E;
+  r.kap = r.kap + op.pscale[da] * (TWO ? rc + s * re : rc);
+  r.kapE = TWO ? op.pscale[da] * re : 0.0;
+  r.dval = (TWO ? mdc + s * mde : mdc) * dl;
+  if (pdiff) {
+    double kxd = r.kap * op.ddiag[lane];
+    kxd = kxd * dl;
+    r.dval = r.dval + kxd;
+  }
+  r.dE = TWO ? (mde + r.kapE * op.ddiag[lane]) * dl : 0.0;
+  return r;
+}
+
+// step_local: step index (use_ctl = 0) or offset from step_ctl[0] (use_ctl = 1,
+// graph replay; steps at or beyond step_ctl[1] do nothing).
+template <bool STAGE2, int KT, bool TWO>
+__global__ void __launch_bounds__(STHREADS) etd2_stage(prh_operator op, prh_schedule sch,
+                                                       prh_state st, int step_local,
+                                                       int use_ctl) {
+  __shared__ double red[SWG][KT][WAVE];
+  int step = step_local;
+  int ridx = -1;
+  if (use_ctl) {
+    step = st.step_ctl[0] + step_local;
+    if (step >= st.step_ctl[1]) return;  // uniform over the grid
+    ridx = step_local;
+  } else {
+    ridx = step - st.rec_first;
+  }
+  const bool record = st.rec_x != nullptr && ridx >= 0 && ridx < st.rec_count;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = threadIdx.x / WAVE;
+  const int K = op.K;
+  const int d = op.d;
+  int gx, gy;
+  wg_coords(op.ntasks, K, KT, gx, gy);
+  const int k0 = gy * KT;
+  const int* tk = op.tasks + 6 * ((int64_t)gx * SWG + wv);
+  const int da = tk[0];
+  const bool active = lane < d;
+
+  const double* __restrict__ xin = STAGE2 ? st.a : st.x;
+  const int w = sch.win[step];
+  const double* __restrict__ mvC = op.mvals + (int64_t)w * op.mvals_stride;
+  const double* __restrict__ mvE = TWO ? op.mvals_e + (int64_t)w * op.mvals_stride : nullptr;
+
+  double acc[KT], accE[KT];
+  if (da >= 0) {
+    couple_part<KT, TWO>(op, xin, mvC, mvE, tk[1], tk[2], tk[5], k0, lane, active, acc, accE);
+  } else {
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) acc[kk] = accE[kk] = 0.0;
+  }
+  double sc_[KT];
+#pragma unroll
+  for (int kk = 0; kk < KT; ++kk) {
+    const int k = k0 + kk < K ? k0 + kk : K - 1;
+    sc_[kk] = TWO ? op.escale[k] : 0.0;
+    if (TWO) acc[kk] = acc[kk] + sc_[kk] * accE[kk];
+  }
+  reduce_parts<KT>(red, wv, lane, tk[3], tk[4], acc);
+  if (da < 0 || wv != tk[3]) return;
 
   // ---- epilogue: row block da, bins = lanes, KT right-hand sides
   const double dl = sch.dldz[step];
   const double hh = sch.h[step];
   const bool pdiff = (op.flags & PRH_F_PARTIAL_DIFF) != 0;
   const bool inj = (op.flags & PRH_F_INJECTION) != 0 && op.src_slot[da] >= 0;
-  double kap = 0.0;
-  double dval = 0.0;
-  if (active) {
-    kap = sch.hadia[step] * op.egrid[lane];
-    kap = kap + op.pscale[da] * op.rpair[(int64_t)w * d + lane];
-    dval = mv[op.zslots + (int64_t)da * d + lane] * dl;
-    if (pdiff) {
-      double kxd = kap * op.ddiag[lane];
-      kxd = kxd * dl;
-      dval = dval + kxd;
-    }
-  }
-  // diagonal phi factors (solvers/etd2.py:329-390)
-  const double hd = dval * hh;
-  const double e = exp(hd);
-  const double ahd = fabs(hd);
-  double phi;
-  if (!STAGE2) {
-    phi = (ahd > PHI1_SMALL) ? (e - 1.0) / hd : (hd * (1.0 / 6.0) + 0.5) * hd + 1.0;
-  } else {
-    phi = (ahd > PHI2_SMALL) ? ((e - 1.0) - hd) / (hd * hd)
-                             : (hd * (1.0 / 24.0) + (1.0 / 6.0)) * hd + 0.5;
-  }
   double dcf[7];
 #pragma unroll
   for (int o = 0; o < 7; ++o) dcf[o] = (active && pdiff) ? op.dcoef[lane * 7 + o] : 0.0;
   const double finj = sch.finj[step];
   const int slot = inj ? op.src_slot[da] : 0;
   const int J = op.nsrc;
+  const int64_t nk = (int64_t)op.nspec * K * d;
 
 #pragma unroll
   for (int kk = 0; kk < KT; ++kk) {
     const int k = k0 + kk;
     if (k >= K) break;  // uniform over the wave
+    RowCoef rc{0.0, 0.0, 0.0, 0.0};
+    if (active) rc = row_coef<TWO>(op, sch, step, w, da, lane, sc_[kk], pdiff);
     const int64_t own = ((int64_t)da * K + k) * d + lane;
     const double xown = active ? xin[own] : 0.0;
     double out = acc[kk];
     if (pdiff) {
-      const double kx = kap * xown;
+      const double kx = rc.kap * xown;
       double dx = 0.0;
 #pragma unroll
       for (int o = -3; o <= 3; ++o) {
@@ -352,19 +426,196 @@ __global__ void __launch_bounds__(STHREADS) etd2_stage(prh_operator op, prh_sche
       out = out + finj * bsum;
     }
     if (!active) continue;
+    const double hd = rc.dval * hh;
+    const double e = exp(hd);
     if (!STAGE2) {
+      const double phi = phi1_of(hd, e);
       double a = e * xown;
       double sc = phi * out;
       sc = sc * hh;
       a = a + sc;
       st.a[own] = a;
       st.fphi[own] = out;
+      if (record) {
+        st.rec_x[ridx * nk + own] = xown;
+        st.rec_a[ridx * nk + own] = a;
+        st.rec_fx[ridx * nk + own] = out;
+      }
     } else {
+      const double phi = phi2_of(hd, e);
       double sc = out - st.fphi[own];
       sc = sc * hh;
       sc = sc * phi;
       st.x[own] = xown + sc;
+      if (record) st.rec_fa[ridx * nk + own] = out;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint ETD2 stages (no counterpart in the reference).  Forward step:
+//   a = e x + h phi1 F(x),  x' = a + h phi2 L (a - x),  F(y) = L y + b,
+//   L y = dl (M y + D (kappa y)).
+// Given lam' = dJ/dx' and u = h phi2 lam':
+//   stage 1: lam_a = lam' + L^T u,  v = h phi1 lam_a - u,
+//            dJ/db += h phi1 lam_a (source accumulators),
+//            dJ/ds += dl <(M_E + D kappa_E)^T u, a> + <dJ/dd, dd/ds>;
+//   stage 2: lam = e lam_a + L^T v,  dJ/ds += dl <(M_E + D kappa_E)^T v, x>,
+//            and u for the previous step.
+// L^T y = dl (M^T y + kappa (D^T y)); the operator passed here is the
+// transposed block layout and the transposed FD stencil.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double dphi1(double z, double e, double phi1) {
+  // d phi1 / dz = (e - phi1) / z, series near 0
+  return (fabs(z) < 1e-2) ? 0.5 + z * (1.0 / 3.0 + z * (1.0 / 8.0 + z * (1.0 / 30.0)))
+                          : (e - phi1) / z;
+}
+__device__ __forceinline__ double dphi2(double z, double phi1, double phi2) {
+  // d phi2 / dz = (phi1 - 2 phi2) / z, series near 0
+  return (fabs(z) < 1e-2) ? 1.0 / 6.0 + z * (1.0 / 12.0 + z * (1.0 / 40.0 + z * (1.0 / 180.0)))
+                          : (phi1 - 2.0 * phi2) / z;
+}
+
+template <bool ASTAGE2, int KT, bool TWO>
+__global__ void __launch_bounds__(STHREADS) etd2_adjoint(prh_operator op, prh_schedule sch,
+                                                         prh_adjoint adj, int step) {
+  __shared__ double red[SWG][KT][WAVE];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = threadIdx.x / WAVE;
+  const int K = op.K;
+  const int d = op.d;
+  int gx, gy;
+  wg_coords(op.ntasks, K, KT, gx, gy);
+  const int k0 = gy * KT;
+  const int* tk = op.tasks + 6 * ((int64_t)gx * SWG + wv);
+  const int r = tk[0];
+  const bool active = lane < d;
+  const double* __restrict__ yin = ASTAGE2 ? adj.v : adj.u;
+  const int w = sch.win[step];
+  const double* __restrict__ mvC = op.mvals + (int64_t)w * op.mvals_stride;
+  const double* __restrict__ mvE = TWO ? op.mvals_e + (int64_t)w * op.mvals_stride : nullptr;
+
+  double acc[KT], accE[KT];
+  if (r >= 0) {
+    couple_part<KT, TWO>(op, yin, mvC, mvE, tk[1], tk[2], tk[5], k0, lane, active, acc, accE);
+  } else {
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) acc[kk] = accE[kk] = 0.0;
+  }
+  reduce_parts<KT>(red, wv, lane, tk[3], tk[4], acc);
+  if (TWO) {
+    __syncthreads();  // the lead waves have read red before it is reused
+    reduce_parts<KT>(red, wv, lane, tk[3], tk[4], accE);
+  }
+  if (r < 0 || wv != tk[3]) return;
+
+  const double dl = sch.dldz[step];
+  const double hh = sch.h[step];
+  const bool pdiff = (op.flags & PRH_F_PARTIAL_DIFF) != 0;
+  double dcf[7];
+#pragma unroll
+  for (int o = 0; o < 7; ++o) dcf[o] = (active && pdiff) ? op.dcoef[lane * 7 + o] : 0.0;
+  const int64_t nk = (int64_t)op.nspec * K * d;
+  const int ridx = step - adj.rec_first;
+  const bool want_s = TWO && adj.gdot != nullptr;
+  const bool inj = !ASTAGE2 && adj.gsrc0 != nullptr && (op.flags & PRH_F_INJECTION) != 0 &&
+                   op.src_slot[r] >= 0;
+
+#pragma unroll
+  for (int kk = 0; kk < KT; ++kk) {
+    const int k = k0 + kk;
+    if (k >= K) break;
+    const double s = TWO ? op.escale[k] : 0.0;
+    RowCoef rc{0.0, 0.0, 0.0, 0.0};
+    if (active) rc = row_coef<TWO>(op, sch, step, w, r, lane, s, pdiff);
+    const int64_t own = ((int64_t)r * K + k) * d + lane;
+    const double yown = active ? yin[own] : 0.0;
+    // transposed FD stencil D^T y at this row, then kappa-scaled
+    double q = 0.0;
+    if (pdiff) {
+#pragma unroll
+      for (int o = -3; o <= 3; ++o) {
+        if (o == 0) continue;
+        const double yo = shfl_or_zero(yown, lane + o, d);
+        if (dcf[o + 3] != 0.0) q += dcf[o + 3] * yo;
+      }
+    }
+    const double mty = TWO ? acc[kk] + s * accE[kk] : acc[kk];
+    const double t = (mty + rc.kap * q) * dl;
+    if (!active) continue;
+    const double hd = rc.dval * hh;
+    const double e = exp(hd);
+    const double p1 = phi1_of(hd, e);
+    const double p2 = phi2_of(hd, e);
+    double gs = 0.0;
+    if (!ASTAGE2) {
+      const double lamp = adj.lam[own];
+      const double lam_a = lamp + t;
+      const double wv_ = hh * p1 * lam_a;
+      adj.lam_a[own] = lam_a;
+      adj.v[own] = wv_ - yown;
+      if (inj) {
+        const int slot = op.src_slot[r];
+        const double f = sch.finj[step];
+        const double lz = adj.lnz1[step];
+        for (int j = 0; j < op.nsrc; ++j) {
+          const double c = sch.src_coef[((int64_t)step * op.nsrc + j) * K + k];
+          const int64_t gi = (((int64_t)slot * op.nsrc + j) * K + k) * d + lane;
+          adj.gsrc0[gi] += f * c * wv_;
+          adj.gsrc1[gi] += f * c * lz * wv_;
+        }
+      }
+      if (want_s) {
+        const double xa = adj.rec_a[ridx * nk + own];
+        const double xx = adj.rec_x[ridx * nk + own];
+        const double fx = adj.rec_fx[ridx * nk + own];
+        const double fa = adj.rec_fa[ridx * nk + own];
+        gs = dl * (accE[kk] + rc.kapE * q) * xa;
+        const double dJdd = lam_a * (hh * e * xx + hh * hh * dphi1(hd, e, p1) * fx) +
+                            lamp * (hh * hh * dphi2(hd, p1, p2) * (fa - fx));
+        gs += dJdd * rc.dE;
+      }
+    } else {
+      const double lam_a = adj.lam_a[own];
+      const double lam = e * lam_a + t;
+      adj.lam[own] = lam;
+      if (want_s) {
+        const double xx = adj.rec_x[ridx * nk + own];
+        gs = dl * (accE[kk] + rc.kapE * q) * xx;
+      }
+      // u of the previous step: h_{k-1} phi2(h_{k-1} d_{k-1}) lam
+      if (step > 0) {
+        const int sp = step - 1;
+        const int wp = sch.win[sp];
+        const RowCoef rp = row_coef<TWO>(op, sch, sp, wp, r, lane, s, pdiff);
+        const double hdp = rp.dval * sch.h[sp];
+        const double ep = exp(hdp);
+        adj.u[own] = sch.h[sp] * phi2_of(hdp, ep) * lam;
+      }
+    }
+    if (want_s) adj.gdot[own] += gs;
+  }
+}
+
+// u = h phi2(h d) lam for the last step of the sweep (start of the adjoint).
+template <bool TWO>
+__global__ void __launch_bounds__(STHREADS) adjoint_seed(prh_operator op, prh_schedule sch,
+                                                         prh_adjoint adj, int step) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = threadIdx.x / WAVE;
+  const int K = op.K;
+  const int d = op.d;
+  const int* tk = op.tasks + 6 * ((int64_t)blockIdx.x * SWG + wv);
+  const int r = tk[0];
+  if (r < 0 || wv != tk[3] || lane >= d) return;
+  const bool pdiff = (op.flags & PRH_F_PARTIAL_DIFF) != 0;
+  const int w = sch.win[step];
+  for (int k = 0; k < K; ++k) {
+    const double s = TWO ? op.escale[k] : 0.0;
+    const RowCoef rc = row_coef<TWO>(op, sch, step, w, r, lane, s, pdiff);
+    const double hd = rc.dval * sch.h[step];
+    const int64_t own = ((int64_t)r * K + k) * d + lane;
+    adj.u[own] = sch.h[step] * phi2_of(hd, exp(hd)) * adj.lam[own];
   }
 }
 
@@ -390,30 +641,63 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
     return set_error(std::string(who) + ": kt must be 1, 2, 4 or 8");
   if ((int64_t)(op->ntasks / SWG) * ((op->K + op->kt - 1) / op->kt) > 0x7fffffff)
     return set_error(std::string(who) + ": grid too large");
+  if ((op->mvals_e == nullptr) != (op->escale == nullptr) ||
+      (op->mvals_e != nullptr && op->rpair_e == nullptr))
+    return set_error(std::string(who) + ": mvals_e, rpair_e and escale go together");
   return 0;
+}
+
+template <bool S2, bool TWO>
+void launch_stage_two(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
+                      int step_local, int use_ctl, hipStream_t s) {
+  const int64_t nwg = (int64_t)(op.ntasks / SWG) * ((op.K + op.kt - 1) / op.kt);
+  const dim3 grid((unsigned)nwg);
+  switch (op.kt) {
+    case 1:
+      hipLaunchKernelGGL((etd2_stage<S2, 1, TWO>), grid, dim3(STHREADS), 0, s, op, sch, st,
+                         step_local, use_ctl);
+      break;
+    case 2:
+      hipLaunchKernelGGL((etd2_stage<S2, 2, TWO>), grid, dim3(STHREADS), 0, s, op, sch, st,
+                         step_local, use_ctl);
+      break;
+    case 4:
+      hipLaunchKernelGGL((etd2_stage<S2, 4, TWO>), grid, dim3(STHREADS), 0, s, op, sch, st,
+                         step_local, use_ctl);
+      break;
+    default:
+      hipLaunchKernelGGL((etd2_stage<S2, 8, TWO>), grid, dim3(STHREADS), 0, s, op, sch, st,
+                         step_local, use_ctl);
+      break;
+  }
 }
 
 template <bool S2>
 void launch_stage_kt(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
                      int step_local, int use_ctl, hipStream_t s) {
+  if (op.mvals_e)
+    launch_stage_two<S2, true>(op, sch, st, step_local, use_ctl, s);
+  else
+    launch_stage_two<S2, false>(op, sch, st, step_local, use_ctl, s);
+}
+
+template <bool S2, bool TWO>
+void launch_adjoint_two(const prh_operator& op, const prh_schedule& sch, const prh_adjoint& adj,
+                        int step, hipStream_t s) {
   const int64_t nwg = (int64_t)(op.ntasks / SWG) * ((op.K + op.kt - 1) / op.kt);
   const dim3 grid((unsigned)nwg);
   switch (op.kt) {
     case 1:
-      hipLaunchKernelGGL((etd2_stage<S2, 1>), grid, dim3(STHREADS), 0, s, op, sch, st, step_local,
-                         use_ctl);
+      hipLaunchKernelGGL((etd2_adjoint<S2, 1, TWO>), grid, dim3(STHREADS), 0, s, op, sch, adj, step);
       break;
     case 2:
-      hipLaunchKernelGGL((etd2_stage<S2, 2>), grid, dim3(STHREADS), 0, s, op, sch, st, step_local,
-                         use_ctl);
+      hipLaunchKernelGGL((etd2_adjoint<S2, 2, TWO>), grid, dim3(STHREADS), 0, s, op, sch, adj, step);
       break;
     case 4:
-      hipLaunchKernelGGL((etd2_stage<S2, 4>), grid, dim3(STHREADS), 0, s, op, sch, st, step_local,
-                         use_ctl);
+      hipLaunchKernelGGL((etd2_adjoint<S2, 4, TWO>), grid, dim3(STHREADS), 0, s, op, sch, adj, step);
       break;
     default:
-      hipLaunchKernelGGL((etd2_stage<S2, 8>), grid, dim3(STHREADS), 0, s, op, sch, st, step_local,
-                         use_ctl);
+      hipLaunchKernelGGL((etd2_adjoint<S2, 8, TWO>), grid, dim3(STHREADS), 0, s, op, sch, adj, step);
       break;
   }
 }
@@ -430,7 +714,7 @@ struct prh_graph {
 
 extern "C" {
 
-int32_t prh_version(void) { return 2; }
+int32_t prh_version(void) { return 3; }
 
 const char* prh_last_error(void) { return g_last_error.c_str(); }
 
@@ -486,6 +770,60 @@ int32_t prh_etd2_stage(const prh_operator* op, const prh_schedule* sch, const pr
   else
     launch_stage_kt<true>(*op, *sch, *st, step, 0, s);
   return check_hip(hipGetLastError(), "prh_etd2_stage launch");
+}
+
+int32_t prh_adjoint_seed(const prh_operator* opT, const prh_schedule* sch,
+                         const prh_adjoint* adj, int32_t step, void* stream) {
+  if (!adj) return set_error("prh_adjoint_seed: null adjoint");
+  prh_state dummy{};
+  if (int32_t e = validate(opT, sch, &dummy, "prh_adjoint_seed")) return e;
+  if (step < 0 || step >= sch->nsteps) return set_error("prh_adjoint_seed: step outside the schedule");
+  if (!adj->lam || !adj->u) return set_error("prh_adjoint_seed: lam and u are required");
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid((unsigned)(opT->ntasks / SWG));
+  if (opT->mvals_e)
+    hipLaunchKernelGGL((adjoint_seed<true>), grid, dim3(STHREADS), 0, s, *opT, *sch, *adj, step);
+  else
+    hipLaunchKernelGGL((adjoint_seed<false>), grid, dim3(STHREADS), 0, s, *opT, *sch, *adj, step);
+  return check_hip(hipGetLastError(), "prh_adjoint_seed launch");
+}
+
+int32_t prh_adjoint_step(const prh_operator* opT, const prh_schedule* sch, const prh_adjoint* adj,
+                         int32_t step, void* stream) {
+  if (!adj) return set_error("prh_adjoint_step: null adjoint");
+  prh_state dummy{};
+  if (int32_t e = validate(opT, sch, &dummy, "prh_adjoint_step")) return e;
+  if (step < 0 || step >= sch->nsteps) return set_error("prh_adjoint_step: step outside the schedule");
+  if (!adj->lam || !adj->lam_a || !adj->u || !adj->v)
+    return set_error("prh_adjoint_step: lam, lam_a, u and v are required");
+  const bool want_s = opT->mvals_e && adj->gdot;
+  if (want_s) {
+    if (!adj->rec_x || !adj->rec_a || !adj->rec_fx || !adj->rec_fa)
+      return set_error("prh_adjoint_step: the EBL-scale gradient needs the recorded forward step");
+    if (step - adj->rec_first < 0 || step - adj->rec_first >= adj->rec_count)
+      return set_error("prh_adjoint_step: step outside the recorded chunk");
+  }
+  if (adj->gsrc0 && (!adj->gsrc1 || !adj->lnz1))
+    return set_error("prh_adjoint_step: source accumulators need gsrc1 and lnz1");
+  hipStream_t s = (hipStream_t)stream;
+  if (opT->mvals_e) {
+    launch_adjoint_two<false, true>(*opT, *sch, *adj, step, s);
+    launch_adjoint_two<true, true>(*opT, *sch, *adj, step, s);
+  } else {
+    launch_adjoint_two<false, false>(*opT, *sch, *adj, step, s);
+    launch_adjoint_two<true, false>(*opT, *sch, *adj, step, s);
+  }
+  return check_hip(hipGetLastError(), "prh_adjoint_step launch");
+}
+
+int32_t prh_adjoint_steps(const prh_operator* opT, const prh_schedule* sch,
+                          const prh_adjoint* adj, int32_t k_begin, int32_t k_end, void* stream) {
+  if (k_begin < 0 || k_end > (sch ? sch->nsteps : 0) || k_begin > k_end)
+    return set_error("prh_adjoint_steps: step range outside the schedule");
+  for (int32_t k = k_end - 1; k >= k_begin; --k) {
+    if (int32_t e = prh_adjoint_step(opT, sch, adj, k, stream)) return e;
+  }
+  return 0;
 }
 
 int32_t prh_graph_create(const prh_operator* op, const prh_schedule* sch, const prh_state* st,

@@ -45,11 +45,13 @@ class BlockLayout:
         return int(self.off_to_M.size)
 
 
-def build_block_layout(indptr, indices, m0, nspec, d, keep_off=True):
+def build_block_layout(indptr, indices, m0, nspec, d, keep_off=True, data_index=None):
     """Block-band layout from a CSR pattern and first-window values ``m0``.
 
     ``keep_off=False`` drops every off-diagonal entry (photo-hadronic
-    coupling switched off).
+    coupling switched off).  ``data_index`` maps the positions of this CSR to
+    the rows of the fold kernel (identity when None); it is how the
+    transposed layout points back into the forward kernel.
     """
     n = nspec * d
     indptr = np.asarray(indptr, dtype=np.int64)
@@ -90,9 +92,14 @@ def build_block_layout(indptr, indices, m0, nspec, d, keep_off=True):
         slot = np.zeros(0, dtype=np.int64)
     if zslots >= 2**31:
         raise ValueError("coupling matrix too large for 32-bit block offsets")
+    src = k_off if data_index is None else np.asarray(data_index, dtype=np.int64)[k_off]
+    dsrc = diag_to_M.copy()
+    if data_index is not None:
+        ok = dsrc >= 0
+        dsrc[ok] = np.asarray(data_index, dtype=np.int64)[dsrc[ok]]
     sel = np.full(zslots + n, -1, dtype=np.int64)
-    sel[slot] = k_off
-    sel[zslots:] = diag_to_M
+    sel[slot] = src
+    sel[zslots:] = dsrc
     blocks = np.stack([blk_mo, olo, nslot, voff], axis=1).astype(np.int32).reshape(nb, 4)
     brow_ptr = np.searchsorted(blk_da, np.arange(nspec + 1), side="left").astype(np.int64)
     return BlockLayout(
@@ -182,6 +189,73 @@ def build_tasks(layout: BlockLayout, part_cost=96, wg_waves=16):
         tasks.extend(g)
         tasks.extend([[-1, 0, 0, 0, 0, 0]] * (wg_waves - len(g)))
     return np.asarray(tasks, dtype=np.int32).reshape(-1, 6)
+
+
+def transposed_layout(indptr, indices, m0, nspec, d, keep_off=True):
+    """Block-band layout of M^T whose selection map points at rows of the
+    forward fold kernel (the entry (r, c) of M feeds slot (c, r) of M^T)."""
+    import scipy.sparse as sp
+
+    n = nspec * d
+    nnz = len(indices)
+    carrier = sp.csr_matrix(
+        (np.arange(nnz, dtype=np.float64) + 1.0, np.asarray(indices), np.asarray(indptr)), shape=(n, n)
+    )
+    mt = carrier.T.tocsr()
+    mt.sort_indices()
+    idx = (mt.data - 1.0).astype(np.int64)
+    return build_block_layout(mt.indptr, mt.indices, np.asarray(m0)[idx], nspec, d, keep_off,
+                              data_index=idx)
+
+
+class WindowLayout:
+    """Per-window value layout: [diagonal n][forward band slots][transposed band slots].
+
+    ``blocks`` / ``blocks_t`` carry absolute value offsets inside a window;
+    ``sel`` is the row map of the fold kernel for the whole window.
+    """
+
+    def __init__(self, fwd: BlockLayout, tr: BlockLayout = None):
+        n = fwd.n
+        self.fwd = fwd
+        self.tr = tr
+        self.n = n
+        self.nspec = fwd.nspec
+        self.d = fwd.d
+        self.diag_off = 0
+        self.fwd_off = n
+        self.tr_off = n + fwd.zslots
+        self.stride = n + fwd.zslots + (tr.zslots if tr is not None else 0)
+        self.blocks = fwd.blocks.copy()
+        if self.blocks.size:
+            self.blocks[:, 3] += self.fwd_off
+        parts = [fwd.sel[fwd.zslots:], fwd.sel[: fwd.zslots]]
+        if tr is not None:
+            self.blocks_t = tr.blocks.copy()
+            if self.blocks_t.size:
+                self.blocks_t[:, 3] += self.tr_off
+            parts.append(tr.sel[: tr.zslots])
+        else:
+            self.blocks_t = None
+        self.sel = np.concatenate(parts)
+        if self.stride >= 2**31:
+            raise ValueError("window too large for 32-bit block offsets")
+
+    @property
+    def nnz_off(self):
+        return self.fwd.nnz_off
+
+
+def fd_transposed_tables(block):
+    """Transposed FD table [d, 7]: entry (i, o) = D[i + o, i] for o != 0."""
+    d = block.shape[0]
+    dct = np.zeros((d, 7))
+    for i in range(d):
+        for o in range(-3, 4):
+            j = i + o
+            if o != 0 and 0 <= j < d:
+                dct[i, o + 3] = block[j, i]
+    return dct
 
 
 def fd_tables(block):

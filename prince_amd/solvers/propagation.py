@@ -31,7 +31,14 @@ from .. import _lib
 from ..model.cosmology import H
 from ..model.partial_diff import DifferentialOperator
 from ..model.units import PRINCE_UNITS
-from .layout import build_block_layout, build_tasks, fd_tables
+from .layout import (
+    WindowLayout,
+    build_block_layout,
+    build_tasks,
+    fd_tables,
+    fd_transposed_tables,
+    transposed_layout,
+)
 from .result import UHECRPropagationResult
 
 
@@ -287,10 +294,15 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             run._prh_device_cache = cache
         return cache
 
-    def _layout_for(self, cache, phot0):
-        """Block layout for the first-window photon vector (cached per run)."""
+    def _layout_for(self, cache, phot0, with_transpose=False):
+        """Window layout for the first-window photon vector (cached per run).
+
+        The first window is folded on the device and read back once; the
+        off-diagonal pattern is its non-zero set (the reference's
+        ``eliminate_zeros`` at the first refresh).
+        """
         torch = _torch()
-        key = (phot0.tobytes(), bool(self.enable_photohad_losses))
+        key = (phot0.tobytes(), bool(self.enable_photohad_losses), bool(with_transpose))
         lay = cache.layouts.get(key)
         if lay is not None:
             return lay
@@ -306,16 +318,13 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         )
         m0_host = m0.cpu().numpy()
         ir = self.had_int_rates
-        layout = build_block_layout(
-            ir.coupling_mat.indptr,
-            ir.coupling_mat.indices,
-            m0_host,
-            self.spec_man.nspec,
-            self.prince_run.cr_grid.d,
-            keep_off=bool(self.enable_photohad_losses),
-        )
-        nsel = layout.sel.size
-        sel_dev = torch.from_numpy(layout.sel).to(self.device)
+        args = (ir.coupling_mat.indptr, ir.coupling_mat.indices, m0_host, self.spec_man.nspec,
+                self.prince_run.cr_grid.d, bool(self.enable_photohad_losses))
+        fwd = build_block_layout(*args)
+        tr = transposed_layout(*args) if with_transpose else None
+        wl = WindowLayout(fwd, tr)
+        nsel = wl.sel.size
+        sel_dev = torch.from_numpy(wl.sel).to(self.device)
         B_sel = torch.empty((nsel, cache.dph), dtype=torch.float64, device=self.device)
         _lib.check(
             self._lib.prh_gather_rows(
@@ -324,20 +333,71 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             ),
             "split fold kernel gather",
         )
-        lay = {"layout": layout, "B_sel": B_sel, "m0": m0_host}
+        lay = {"layout": wl, "B_sel": B_sel, "m0": m0_host}
         cache.layouts[key] = lay
         return lay
 
+    # ------------------------------------------------------- photon-field split
+    def set_field_scales(self, component, scales):
+        """Scale one component of the run's combined photon field per right-hand side.
+
+        Column k then sees n(E, z) = n_rest(E, z) + scales[k] * n_component(E, z):
+        the rate matrix, its diagonal and the pair-production rate become
+        ``M_rest + s_k M_component`` (two folds shared by every column).
+        ``component`` indexes ``prince_run.photon_field.model_list``.
+        """
+        pf = self.prince_run.photon_field
+        if not hasattr(pf, "model_list"):
+            raise TypeError("set_field_scales needs a CombinedPhotonField")
+        scales = np.asarray(scales, dtype=np.float64).reshape(-1)
+        if scales.size != self._K:
+            raise ValueError(f"need {self._K} scales, got {scales.size}")
+        self._field_split = (int(component), scales.copy())
+
+    def _split_fields(self):
+        fs = getattr(self, "_field_split", None)
+        if fs is None:
+            return None
+        idx, scales = fs
+        pf = self.prince_run.photon_field
+        comp = pf.model_list[idx]
+        rest = [m for i, m in enumerate(pf.model_list) if i != idx]
+        return rest, comp, scales
+
     def _refresh_z_caches(self, zs):
-        """Host inputs of every refresh window: photon vectors and pair rates."""
+        """Host inputs of every refresh window: photon vectors and pair rates.
+
+        Returns (phot, rpair, phot_e, rpair_e); the last two are None unless a
+        per-column field scale is set.
+        """
         ir = self.had_int_rates
-        phot = np.stack([ir.photon_vector(z) for z in zs]).astype(np.float64)
+        pair = self.pair_loss_rates_grid
         d = self.prince_run.cr_grid.d
+        split = self._split_fields()
+        if split is None:
+            phot = np.stack([ir.photon_vector(z) for z in zs]).astype(np.float64)
+            if self.enable_pairprod_losses:
+                rpair = np.stack([pair.rate_single(z) for z in zs])
+            else:
+                rpair = np.zeros((len(zs), d))
+            return phot, rpair, None, None
+        rest, comp, _ = split
+
+        class _Sum:
+            def get_photon_density(self_, E, z):
+                E = np.atleast_1d(np.asarray(E, dtype=np.float64))
+                return np.sum([m.get_photon_density(E, z) for m in rest], axis=0)
+
+        base = _Sum()
+        phot = np.stack([ir.photon_vector(z, pfield=base) for z in zs]).astype(np.float64)
+        phot_e = np.stack([ir.photon_vector(z, pfield=comp) for z in zs]).astype(np.float64)
         if self.enable_pairprod_losses:
-            rpair = np.stack([self.pair_loss_rates_grid.rate_single(z) for z in zs])
+            rpair = np.stack([pair.rate_single(z, pfield=base) for z in zs])
+            rpair_e = np.stack([pair.rate_single(z, pfield=comp) for z in zs])
         else:
             rpair = np.zeros((len(zs), d))
-        return phot, rpair
+            rpair_e = np.zeros((len(zs), d))
+        return phot, rpair, phot_e, rpair_e
 
     def _schedule_arrays(self, z_grid, col_sources):
         nsteps = len(z_grid) - 1
@@ -373,10 +433,10 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             for j, src in enumerate(srcs):
                 g = np.asarray(src.injection_grid).reshape(nspec, d)
                 grid[: slots.size, j, kcol, :] = g[slots]
-        return src_slot, grid
+        return src_slot, grid, slots
 
-    def _ensure_apply_F(self, z_grid):
-        """Build the device plan of a solve: operator, schedule and buffers."""
+    def _ensure_apply_F(self, z_grid, with_adjoint=False):
+        """Build the device plan of a solve: operator(s), schedule and buffers."""
         torch = _torch()
         t0 = time.perf_counter()
         dev = self.device
@@ -386,21 +446,21 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             raise NotImplementedError("the HIP stage kernels support at most 64 energy bins")
         cache = self._device_cache()
         win, zs = refresh_schedule(z_grid, self.recomp_z_threshold)
-        phot, rpair = self._refresh_z_caches(zs)
-        lay = self._layout_for(cache, phot[0])
-        layout = lay["layout"]
+        phot, rpair, phot_e, rpair_e = self._refresh_z_caches(zs)
+        two = phot_e is not None
+        phot0 = phot[0] + phot_e[0] if two else phot[0]
+        lay = self._layout_for(cache, phot0, with_transpose=with_adjoint)
+        wl = lay["layout"]
         nwin = len(zs)
-        stride = layout.zslots + layout.n
+        stride = wl.stride
         stream = torch.cuda.current_stream(dev).cuda_stream
-        phot_dev = torch.from_numpy(np.ascontiguousarray(phot)).to(dev)
-        mvals = torch.empty((nwin, stride), dtype=torch.float64, device=dev)
         t1 = time.perf_counter()
-        tasks = build_tasks(layout, self.part_cost_for(K), _lib.WG_WAVES)
+        tasks = build_tasks(wl.fwd, self.part_cost_for(K), _lib.WG_WAVES)
         kt = self.kt_for(K)
         col_sources = self._column_sources()
         has_src = any(len(c) for c in col_sources) and self.enable_injection_jacobian
         h, dldz, hadia, finj, coef, J = self._schedule_arrays(z_grid, col_sources)
-        src_slot, src_grid = self._source_tables(col_sources, J)
+        src_slot, src_grid, slots = self._source_tables(col_sources, J)
         dcoef, ddiag = self._D_tables
         pscale = np.asarray(self.pair_loss_rates_grid.species_scale, dtype=np.float64)
 
@@ -413,10 +473,13 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             flags |= _lib.PRH_F_PARTIAL_DIFF
         if has_src:
             flags |= _lib.PRH_F_INJECTION
+        blocks = wl.blocks if wl.blocks.size else np.zeros((1, 4))
         bufs = {
             "tasks": dv(tasks, np.int32),
-            "blocks": dv(layout.blocks if layout.blocks.size else np.zeros((1, 4)), np.int32),
-            "mvals": mvals,
+            "blocks": dv(blocks, np.int32),
+            "mvals": torch.empty((nwin, stride), dtype=torch.float64, device=dev),
+            "phot": dv(phot, np.float64),
+            "B_sel": lay["B_sel"],
             "egrid": dv(self.prince_run.cr_grid.grid, np.float64),
             "dcoef": dv(dcoef, np.float64),
             "ddiag": dv(ddiag, np.float64),
@@ -434,15 +497,34 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             "fphi": torch.zeros((nspec, K, d), dtype=torch.float64, device=dev),
             "step_ctl": torch.zeros(2, dtype=torch.int32, device=dev),
         }
-        op = _lib.PrhOperator(
-            nspec=nspec, d=d, K=K, ntasks=tasks.shape[0],
-            tasks=bufs["tasks"].data_ptr(), blocks=bufs["blocks"].data_ptr(),
-            mvals_stride=stride, zslots=layout.zslots, mvals=mvals.data_ptr(),
-            egrid=bufs["egrid"].data_ptr(), dcoef=bufs["dcoef"].data_ptr(),
-            ddiag=bufs["ddiag"].data_ptr(), pscale=bufs["pscale"].data_ptr(),
-            rpair=bufs["rpair"].data_ptr(), src_slot=bufs["src_slot"].data_ptr(),
-            nsrc=J, flags=flags, src_grid=bufs["src_grid"].data_ptr(), kt=kt, pad_=0,
-        )
+        if two:
+            bufs["mvals_e"] = torch.empty((nwin, stride), dtype=torch.float64, device=dev)
+            bufs["phot_e"] = dv(phot_e, np.float64)
+            bufs["rpair_e"] = dv(rpair_e, np.float64)
+            bufs["escale"] = dv(self._field_split[1], np.float64)
+
+        def make_op(tk, blk, dcf):
+            return _lib.PrhOperator(
+                nspec=nspec, d=d, K=K, ntasks=tk.shape[0], tasks=tk.data_ptr(),
+                blocks=blk.data_ptr(), mvals_stride=stride, diag_off=wl.diag_off,
+                mvals=bufs["mvals"].data_ptr(), egrid=bufs["egrid"].data_ptr(),
+                dcoef=dcf.data_ptr(), ddiag=bufs["ddiag"].data_ptr(),
+                pscale=bufs["pscale"].data_ptr(), rpair=bufs["rpair"].data_ptr(),
+                src_slot=bufs["src_slot"].data_ptr(), nsrc=J, flags=flags,
+                src_grid=bufs["src_grid"].data_ptr(), kt=kt, pad_=0,
+                mvals_e=bufs["mvals_e"].data_ptr() if two else None,
+                rpair_e=bufs["rpair_e"].data_ptr() if two else None,
+                escale=bufs["escale"].data_ptr() if two else None,
+            )
+
+        op = make_op(bufs["tasks"], bufs["blocks"], bufs["dcoef"])
+        opT = None
+        if with_adjoint:
+            tasks_t = build_tasks(wl.tr, self.part_cost_for(K), _lib.WG_WAVES)
+            bufs["tasks_t"] = dv(tasks_t, np.int32)
+            bufs["blocks_t"] = dv(wl.blocks_t if wl.blocks_t.size else np.zeros((1, 4)), np.int32)
+            bufs["dcoef_t"] = dv(fd_transposed_tables(self.diff_operator.block), np.float64)
+            opT = make_op(bufs["tasks_t"], bufs["blocks_t"], bufs["dcoef_t"])
         sch = _lib.PrhSchedule(
             nsteps=len(h), pad_=0, h=bufs["h"].data_ptr(), dldz=bufs["dldz"].data_ptr(),
             hadia=bufs["hadia"].data_ptr(), finj=bufs["finj"].data_ptr(),
@@ -451,15 +533,15 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         x = torch.zeros((nspec, K, d), dtype=torch.float64, device=dev)
         st = _lib.PrhState(
             x=x.data_ptr(), a=bufs["a"].data_ptr(), fphi=bufs["fphi"].data_ptr(),
-            step_ctl=bufs["step_ctl"].data_ptr(),
+            step_ctl=bufs["step_ctl"].data_ptr(), rec_x=None, rec_a=None, rec_fx=None,
+            rec_fa=None, rec_first=0, rec_count=0,
         )
-        bufs["phot"] = phot_dev
-        bufs["B_sel"] = lay["B_sel"]
         self._plan = {
-            "op": op, "sch": sch, "st": st, "x": x, "bufs": bufs, "nsteps": len(h),
-            "layout": layout, "nwin": nwin, "zs": zs, "tasks": tasks, "kt": kt,
+            "op": op, "opT": opT, "sch": sch, "st": st, "x": x, "bufs": bufs,
+            "nsteps": len(h), "layout": wl, "nwin": nwin, "zs": zs, "tasks": tasks, "kt": kt,
             "stream": stream, "stride": stride, "dph": cache.dph, "K": K,
-            "fold_rows": stride, "graph": None,
+            "fold_rows": stride, "graph": None, "two": two, "z_grid": z_grid,
+            "src_slots": slots, "J": J, "win": win,
         }
         if getattr(self.backend, "use_hip_graphs", True) and len(h) >= self.graph_chunk:
             self._plan["graph"] = _lib.StepGraph(op, sch, st, self.graph_chunk)
@@ -468,25 +550,27 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         return self._plan
 
     # ---------------------------------------------------------------- device run
-    def prepare(self, dz=1e-3):
+    def prepare(self, dz=1e-3, with_adjoint=False):
         """Host-side setup of a solve; every input ends up resident in HBM."""
         z_grid = z_grid_for(self.initial_z, self.final_z, dz)
         self._reset_for_solve()
         self._ensure_D_split()
-        return self._ensure_apply_F(z_grid)
+        return self._ensure_apply_F(z_grid, with_adjoint=with_adjoint)
 
     def fold_windows(self, plan):
-        """Rate fold of every refresh window (one MFMA GEMM) into the plan's mvals."""
+        """Rate fold of every refresh window (one MFMA GEMM per field) into mvals."""
         b = plan["bufs"]
-        _lib.check(
-            self._lib.prh_fold_gemm(
-                b["B_sel"].data_ptr(), plan["fold_rows"], plan["dph"], b["phot"].data_ptr(),
-                plan["nwin"], b["mvals"].data_ptr(), plan["stride"], plan["stream"],
-            ),
-            "window fold",
-        )
-        if not self.enable_photohad_losses:
-            b["mvals"].zero_()
+        pairs = [("phot", "mvals")] + ([("phot_e", "mvals_e")] if plan["two"] else [])
+        for ph, mv in pairs:
+            _lib.check(
+                self._lib.prh_fold_gemm(
+                    b["B_sel"].data_ptr(), plan["fold_rows"], plan["dph"], b[ph].data_ptr(),
+                    plan["nwin"], b[mv].data_ptr(), plan["stride"], plan["stream"],
+                ),
+                "window fold",
+            )
+            if not self.enable_photohad_losses:
+                b[mv].zero_()
 
     def execute(self, plan=None, k_begin=0, k_end=None):
         """Device part of a solve: zero state, fold, ETD2 steps (asynchronous)."""
@@ -498,8 +582,8 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         self._launch_steps(plan, k_begin, k_end)
         return plan["x"]
 
-    def _launch_steps(self, plan, k_begin, k_end):
-        if plan["graph"] is not None:
+    def _launch_steps(self, plan, k_begin, k_end, use_graph=True):
+        if use_graph and plan["graph"] is not None:
             plan["graph"].run(k_begin, k_end, plan["stream"])
             return
         _lib.check(
@@ -509,6 +593,115 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             ),
             "ETD2 steps",
         )
+
+    # ---------------------------------------------------------------- adjoint
+    def forward_checkpointed(self, plan, chunk=None):
+        """Forward solve keeping the state at every ``chunk``-th step (default: the
+        graph chunk, 64) for the adjoint sweep; returns the checkpoint tensor."""
+        torch = _torch()
+        chunk = chunk or self.graph_chunk
+        nsteps = plan["nsteps"]
+        nchk = (nsteps + chunk - 1) // chunk
+        ck = plan.get("ckpt")
+        if ck is None or ck.shape[0] != nchk:
+            ck = torch.empty((nchk,) + tuple(plan["x"].shape), dtype=torch.float64,
+                             device=self.device)
+            plan["ckpt"] = ck
+        plan["ckpt_chunk"] = chunk
+        plan["x"].zero_()
+        self.fold_windows(plan)
+        for c in range(nchk):
+            ck[c].copy_(plan["x"])
+            self._launch_steps(plan, c * chunk, min((c + 1) * chunk, nsteps),
+                               use_graph=(chunk == self.graph_chunk))
+        return ck
+
+    def _adjoint_buffers(self, plan):
+        torch = _torch()
+        ab = plan.get("adj_bufs")
+        if ab is not None:
+            return ab
+        dev = self.device
+        shape = tuple(plan["x"].shape)
+        C = plan["ckpt_chunk"]
+        J, K, d = plan["J"], plan["K"], self.prince_run.cr_grid.d
+        nslot = max(1, len(plan["src_slots"]))
+        z0 = plan["z_grid"][:-1]
+        ab = {
+            "lam": torch.zeros(shape, dtype=torch.float64, device=dev),
+            "lam_a": torch.zeros(shape, dtype=torch.float64, device=dev),
+            "u": torch.zeros(shape, dtype=torch.float64, device=dev),
+            "v": torch.zeros(shape, dtype=torch.float64, device=dev),
+            "rec": torch.zeros((4, C) + shape, dtype=torch.float64, device=dev),
+            "gsrc0": torch.zeros((nslot, J, K, d), dtype=torch.float64, device=dev),
+            "gsrc1": torch.zeros((nslot, J, K, d), dtype=torch.float64, device=dev),
+            "gdot": torch.zeros(shape, dtype=torch.float64, device=dev),
+            "lnz1": torch.from_numpy(np.log1p(z0)).to(dev),
+        }
+        rec = ab["rec"]
+        plan["st_rec"] = _lib.PrhState(
+            x=plan["x"].data_ptr(), a=plan["bufs"]["a"].data_ptr(),
+            fphi=plan["bufs"]["fphi"].data_ptr(), step_ctl=plan["bufs"]["step_ctl"].data_ptr(),
+            rec_x=rec[0].data_ptr(), rec_a=rec[1].data_ptr(), rec_fx=rec[2].data_ptr(),
+            rec_fa=rec[3].data_ptr(), rec_first=0, rec_count=C,
+        )
+        plan["adj"] = _lib.PrhAdjoint(
+            lam=ab["lam"].data_ptr(), lam_a=ab["lam_a"].data_ptr(), u=ab["u"].data_ptr(),
+            v=ab["v"].data_ptr(), rec_x=rec[0].data_ptr(), rec_a=rec[1].data_ptr(),
+            rec_fx=rec[2].data_ptr(), rec_fa=rec[3].data_ptr(), rec_first=0, rec_count=C,
+            gsrc0=ab["gsrc0"].data_ptr(), gsrc1=ab["gsrc1"].data_ptr(),
+            lnz1=ab["lnz1"].data_ptr(),
+            gdot=ab["gdot"].data_ptr() if plan["two"] else None,
+        )
+        if getattr(self.backend, "use_hip_graphs", True) and C == self.graph_chunk:
+            plan["rec_graph"] = _lib.StepGraph(plan["op"], plan["sch"], plan["st_rec"], C)
+        else:
+            plan["rec_graph"] = None
+        plan["adj_bufs"] = ab
+        return ab
+
+    def adjoint_sweep(self, plan, lam_final):
+        """Reverse ETD2 sweep from dJ/dx(final) (device tensor, state layout).
+
+        Recomputes each checkpoint chunk forward (recording x, a, F(x), F(a)),
+        then runs the adjoint steps of the chunk backwards.  Returns the
+        accumulator tensors: ``gsrc0`` / ``gsrc1`` ([slot][J][K][d]: dJ/db
+        summed with dl/dz cm2sec * norm * evolution and with an extra
+        ln(1+z)) and ``gdot`` ([nspec][K][d], per-element terms of dJ/ds for
+        the scaled photon-field component), and ``lam`` = dJ/dx(initial).
+        """
+        if plan.get("opT") is None:
+            raise RuntimeError("plan built without the transposed operator (prepare(with_adjoint=True))")
+        if plan.get("ckpt") is None:
+            raise RuntimeError("run forward_checkpointed(plan) first")
+        ab = self._adjoint_buffers(plan)
+        for key in ("gsrc0", "gsrc1", "gdot", "u", "v", "lam_a"):
+            ab[key].zero_()
+        ab["lam"].copy_(lam_final)
+        nsteps = plan["nsteps"]
+        C = plan["ckpt_chunk"]
+        ck = plan["ckpt"]
+        adj = plan["adj"]
+        opT = plan["opT"]
+        sch = plan["sch"]
+        stream = plan["stream"]
+        lib = self._lib
+        _lib.check(lib.prh_adjoint_seed(ctypes.byref(opT), ctypes.byref(sch), ctypes.byref(adj),
+                                        nsteps - 1, stream), "adjoint seed")
+        for c in range((nsteps + C - 1) // C - 1, -1, -1):
+            k0, k1 = c * C, min((c + 1) * C, nsteps)
+            plan["x"].copy_(ck[c])
+            if plan["rec_graph"] is not None:
+                plan["rec_graph"].run(k0, k1, stream)
+            else:
+                st = plan["st_rec"]
+                st.rec_first = k0
+                _lib.check(lib.prh_etd2_steps(ctypes.byref(plan["op"]), ctypes.byref(sch),
+                                              ctypes.byref(st), k0, k1, stream), "recompute")
+            adj.rec_first = k0
+            _lib.check(lib.prh_adjoint_steps(ctypes.byref(opT), ctypes.byref(sch),
+                                             ctypes.byref(adj), k0, k1, stream), "adjoint steps")
+        return ab
 
     def _run_integration(self, state, z_grid, step_hook=None):
         torch = _torch()
