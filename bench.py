@@ -157,15 +157,20 @@ def adjoint_leg(K, torch, log):
     lmi = prob.loss_of_state(prob.forward(theta - h * v * scale, with_adjoint=False)[2], *lp)[0]
     fd = ((lpl - lmi) / (2 * h)).detach().cpu().numpy()
     an = (grad * v * scale).sum(axis=1)
-    rel = float(np.max(np.abs(an - fd) / np.maximum(np.abs(fd), 1e-300)))
+    # instances whose frozen interpolation knots change sign under the step give
+    # NaN differences (the loss is not smooth there); they are left out
+    ok = np.isfinite(fd)
+    rel = float(np.max(np.abs(an[ok] - fd[ok]) / np.maximum(np.abs(fd[ok]), 1e-300))) if ok.any() else None
     log(f"adjoint K={K}: forward {t_f*1e3:.1f} ms, backward {t_b*1e3:.1f} ms, "
-        f"ratio {t_b/t_f:.2f}, directional FD rel dev {rel:.2e}")
+        f"ratio {t_b/t_f:.2f}, directional FD rel dev {rel} on {int(ok.sum())}/{K} instances")
     return {
         "K": K,
         "forward_ms": round(t_f * 1e3, 2),
         "backward_ms": round(t_b * 1e3, 2),
         "ratio": round(t_b / t_f, 3),
         "fd_directional_max_rel_dev": rel,
+        "fd_directional_instances": int(ok.sum()),
+        "grad_finite": bool(np.all(np.isfinite(grad))),
         "params_per_instance": 8,
         "checkpoint_every": 64,
     }
@@ -191,6 +196,13 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         sys.exit(1)
+    # calibration copy of a known size (256 MiB read + 256 MiB written) for
+    # scaling the profiler's FETCH_SIZE / WRITE_SIZE counters
+    calib_src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    calib_dst = torch.empty_like(calib_src)
+    calib_dst.copy_(calib_src)
+    torch.cuda.synchronize()
+    del calib_src, calib_dst
     t0 = time.perf_counter()
     run = workloads.nuclear_run()
     n = run.dim_states
@@ -254,10 +266,12 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": "etd2_stage (stage 1 and 2, batched K=%d)" % K,
+            "kernel": (("etd2_wide + etd2_wide_combine" if bplan.get("wide") else "etd2_stage")
+                       + " (stage 1 and 2, batched K=%d)" % K),
             "bytes_per_launch": nbytes,
             "avg_launch_us": round(avg * 1e6, 2),
         }
+        kernels["stage_layout"] = "wide" if bplan.get("wide") else "narrow"
         kernels["etd2_stage1_us"] = round(t1s * 1e6, 2)
         kernels["etd2_stage2_us"] = round(t2s * 1e6, 2)
         # fold GEMM (MFMA) timing

@@ -71,6 +71,18 @@ typedef struct prh_operator {
   const double* mvals_e;   /* [nwin][mvals_stride] second component or NULL */
   const double* rpair_e;   /* [nwin][d] second component's pair rate or NULL */
   const double* escale;    /* [K] second component's scale per rhs or NULL */
+  /* Wide layout (wide = 1): state [nspec][d][K] with lanes = right-hand
+   * sides, K % 64 == 0, d == 64, one photon field, no step recording;
+   * src_grid is then [nslot][J][d][K].  wtasks [nwtasks][5] = {da,
+   * blk_begin, blk_end, slot, kind}: slot >= 0 stores the part's row sums at
+   * wpart[slot]; wheavy [nheavy][3] = {da, first slot, nparts} lists the rows
+   * whose parts are summed (in part order) and finished by a second kernel. */
+  int32_t wide;
+  int32_t nwtasks;
+  const int32_t* wtasks;
+  int32_t nheavy;
+  int32_t pad2_;
+  const int32_t* wheavy;
 } prh_operator;
 
 #define PRH_F_PARTIAL_DIFF 1 /* continuous-loss stencil term on */
@@ -100,6 +112,7 @@ typedef struct prh_state {
   double* rec_fa;
   int32_t rec_first;
   int32_t rec_count;
+  double* wpart;     /* [nslots][d][K] partial row sums (wide layout) or NULL */
 } prh_state;
 
 /* Adjoint sweep buffers (all [nspec][K][d] unless noted). */

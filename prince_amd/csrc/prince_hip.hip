@@ -153,6 +153,17 @@ __device__ __forceinline__ void couple_part(const prh_operator& op, const double
     accC[kk] = 0.0;
     accE[kk] = 0.0;
   }
+  // the first 64 block descriptors of the part, one per lane, broadcast
+  // with readlane instead of a dependent load per block
+  const int dbase = b_begin;
+  int dmo = 0, dol = 0, dns = 0, dvo = 0;
+  if (b_begin + lane < b_end) {
+    const int* blk = op.blocks + 4 * (b_begin + lane);
+    dmo = blk[0];
+    dol = blk[1];
+    dns = blk[2];
+    dvo = blk[3];
+  }
   if (kind == PART_DIAG) {
     constexpr int UB = Batch<KT, TWO>::DIAG;
     for (int b = b_begin; b < b_end; b += UB) {
@@ -162,9 +173,15 @@ __device__ __forceinline__ void couple_part(const prh_operator& op, const double
       for (int u = 0; u < UB; ++u) {
         const bool valid = b + u < b_end;
         const int bb = valid ? b + u : b_end - 1;
-        const int* blk = op.blocks + 4 * bb;
-        const int mo = blk[0];
-        const int64_t voff = blk[3];
+        int mo, voffi;
+        if (bb - dbase < WAVE) {
+          mo = __builtin_amdgcn_readlane(dmo, bb - dbase);
+          voffi = __builtin_amdgcn_readlane(dvo, bb - dbase);
+        } else {
+          mo = op.blocks[4 * bb];
+          voffi = op.blocks[4 * bb + 3];
+        }
+        const int64_t voff = voffi;
         mc[u] = (active && valid) ? mvC[voff + lane] : 0.0;
         me[u] = (TWO && active && valid) ? mvE[voff + lane] : 0.0;
 #pragma unroll
@@ -192,12 +209,21 @@ __device__ __forceinline__ void couple_part(const prh_operator& op, const double
       for (int u = 0; u < UB; ++u) {
         const bool valid = b + u < b_end;
         const int bb = valid ? b + u : b_end - 1;
-        const int* blk = op.blocks + 4 * bb;
-        const int mo = blk[0];
-        olo[u] = blk[1];
-        ns[u] = valid ? blk[2] : 0;
-        const int64_t voff = blk[3];
-        const int nsl = blk[2];
+        int mo, ol, nsl, voffi;
+        if (bb - dbase < WAVE) {
+          mo = __builtin_amdgcn_readlane(dmo, bb - dbase);
+          ol = __builtin_amdgcn_readlane(dol, bb - dbase);
+          nsl = __builtin_amdgcn_readlane(dns, bb - dbase);
+          voffi = __builtin_amdgcn_readlane(dvo, bb - dbase);
+        } else {
+          mo = op.blocks[4 * bb];
+          ol = op.blocks[4 * bb + 1];
+          nsl = op.blocks[4 * bb + 2];
+          voffi = op.blocks[4 * bb + 3];
+        }
+        olo[u] = ol;
+        ns[u] = valid ? nsl : 0;
+        const int64_t voff = voffi;
 #pragma unroll
         for (int s2 = 0; s2 < NSMAX; ++s2) {
           const int sc = s2 < nsl ? s2 : nsl - 1;
@@ -299,6 +325,30 @@ __device__ __forceinline__ double phi2_of(double hd, double e) {
                                  : (hd * (1.0 / 24.0) + (1.0 / 6.0)) * hd + 0.5;
 }
 
+// Row-local inputs of step `step` for row da (bin = lane).
+struct RowIn {
+  double dl, kadia, pscale, rc, re, mdc, mde, ddiag;
+};
+template <bool TWO>
+__device__ __forceinline__ RowIn row_in(const prh_operator& op, const prh_schedule& sch, int step,
+                                        int w, int da, int lane) {
+  RowIn r;
+  const int d = op.d;
+  r.dl = sch.dldz[step];
+  r.kadia = sch.hadia[step] * op.egrid[lane];
+  r.pscale = op.pscale[da];
+  r.rc = op.rpair[(int64_t)w * d + lane];
+  r.re = TWO ? op.rpair_e[(int64_t)w * d + lane] : 0.0;
+  r.mdc = op.mvals[(int64_t)w * op.mvals_stride + op.diag_off + (int64_t)da * d + lane];
+  r.mde = TWO ? op.mvals_e[(int64_t)w * op.mvals_stride + op.diag_off + (int64_t)da * d + lane]
+              : 0.0;
+  r.ddiag = op.ddiag[lane];
+  return r;
+}
+struct RowCoef;
+template <bool TWO>
+__device__ __forceinline__ RowCoef row_coef_from(const RowIn& ri, double s, bool pdiff);
+
 // Row-local coefficients of step `step` for row da, rhs scale s:
 // kappa, kappa's EBL part, and the diagonal d (propagation.py:1389-1422).
 struct RowCoef {
@@ -329,6 +379,21 @@ __device__ __forceinline__ RowCoef row_coef(const prh_operator& op, const prh_sc
     r.dval = r.dval + kxd;
   }
   r.dE = TWO ? (mde + r.kapE * op.ddiag[lane]) * dl : 0.0;
+  return r;
+}
+
+template <bool TWO>
+__device__ __forceinline__ RowCoef row_coef_from(const RowIn& ri, double s, bool pdiff) {
+  RowCoef r;
+  r.kap = ri.kadia + ri.pscale * (TWO ? ri.rc + s * ri.re : ri.rc);
+  r.kapE = TWO ? ri.pscale * ri.re : 0.0;
+  r.dval = (TWO ? ri.mdc + s * ri.mde : ri.mdc) * ri.dl;
+  if (pdiff) {
+    double kxd = r.kap * ri.ddiag;
+    kxd = kxd * ri.dl;
+    r.dval = r.dval + kxd;
+  }
+  r.dE = TWO ? (ri.mde + r.kapE * ri.ddiag) * ri.dl : 0.0;
   return r;
 }
 
@@ -395,12 +460,20 @@ __global__ void __launch_bounds__(STHREADS) etd2_stage(prh_operator op, prh_sche
   const int J = op.nsrc;
   const int64_t nk = (int64_t)op.nspec * K * d;
 
+  // row-local inputs, loaded once (the stores below may alias them)
+  RowIn ri{};
+  if (active) ri = row_in<TWO>(op, sch, step, w, da, lane);
+  RowCoef rc0{0.0, 0.0, 0.0, 0.0};
+  if (!TWO && active) rc0 = row_coef_from<TWO>(ri, 0.0, pdiff);
+  const double hd0 = rc0.dval * hh;
+  const double e0 = exp(hd0);
+  const double phi0 = STAGE2 ? phi2_of(hd0, e0) : phi1_of(hd0, e0);
 #pragma unroll
   for (int kk = 0; kk < KT; ++kk) {
     const int k = k0 + kk;
     if (k >= K) break;  // uniform over the wave
-    RowCoef rc{0.0, 0.0, 0.0, 0.0};
-    if (active) rc = row_coef<TWO>(op, sch, step, w, da, lane, sc_[kk], pdiff);
+    RowCoef rc = rc0;
+    if (TWO && active) rc = row_coef_from<TWO>(ri, sc_[kk], pdiff);
     const int64_t own = ((int64_t)da * K + k) * d + lane;
     const double xown = active ? xin[own] : 0.0;
     double out = acc[kk];
@@ -427,9 +500,9 @@ __global__ void __launch_bounds__(STHREADS) etd2_stage(prh_operator op, prh_sche
     }
     if (!active) continue;
     const double hd = rc.dval * hh;
-    const double e = exp(hd);
+    const double e = TWO ? exp(hd) : e0;
     if (!STAGE2) {
-      const double phi = phi1_of(hd, e);
+      const double phi = TWO ? phi1_of(hd, e) : phi0;
       double a = e * xown;
       double sc = phi * out;
       sc = sc * hh;
@@ -442,7 +515,7 @@ __global__ void __launch_bounds__(STHREADS) etd2_stage(prh_operator op, prh_sche
         st.rec_fx[ridx * nk + own] = out;
       }
     } else {
-      const double phi = phi2_of(hd, e);
+      const double phi = TWO ? phi2_of(hd, e) : phi0;
       double sc = out - st.fphi[own];
       sc = sc * hh;
       sc = sc * phi;
@@ -630,6 +703,251 @@ __global__ void step_ctl_advance(int32_t* ctl, int32_t chunk) {
   if (threadIdx.x == 0 && blockIdx.x == 0) ctl[0] = ctl[0] + chunk;
 }
 
+// ---------------------------------------------------------------------------
+// Wide stage kernels: lanes = right-hand sides (K a multiple of 64), state
+// layout [species][bin][rhs] -- the reference's multi-RHS (n, K) row-major
+// order.  One wave per (row part, 64-rhs group).  Coupling values are
+// wave-uniform (scalar loads); every block issues 64 independent coalesced
+// state loads; the FD stencil runs in registers over the 64 bins.  Parts of
+// long rows store their partial row sums; etd2_wide_combine adds them in part
+// order and finishes those rows.
+// ---------------------------------------------------------------------------
+constexpr int WTHREADS = 256;
+constexpr int WWAVES = WTHREADS / WAVE;
+
+__device__ __forceinline__ void wide_step(const prh_state& st, int step_local, int use_ctl,
+                                          int& step, bool& live) {
+  step = step_local;
+  live = true;
+  if (use_ctl) {
+    step = st.step_ctl[0] + step_local;
+    live = step < st.step_ctl[1];
+  }
+}
+
+// wave -> (task index, rhs group); XCD-aware when the group count is a multiple of 8
+__device__ __forceinline__ void wide_coords(int ntask, int ngroup, int& task, int& g) {
+  const int wv = threadIdx.x / WAVE;
+  const int nwg_t = (ntask + WWAVES - 1) / WWAVES;
+  const int L = blockIdx.x;
+  int tw;
+  if (ngroup % 8 == 0) {
+    const int j = L >> 3;
+    g = (L & 7) + 8 * (j / nwg_t);
+    tw = j % nwg_t;
+  } else {
+    g = L / nwg_t;
+    tw = L % nwg_t;
+  }
+  task = tw * WWAVES + wv;
+}
+
+template <bool STAGE2>
+__device__ __forceinline__ void wide_epilogue(const prh_operator& op, const prh_schedule& sch,
+                                              const prh_state& st, int step, int w, int da,
+                                              int k, double (&acc)[64], double* lds) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int K = op.K;
+  const bool pdiff = (op.flags & PRH_F_PARTIAL_DIFF) != 0;
+  const double hh = sch.h[step];
+  const double dl = sch.dldz[step];
+  {
+    // lane-parallel coefficients of bin i = lane, shared through LDS
+    const RowIn ri = row_in<false>(op, sch, step, w, da, lane);
+    const RowCoef rc = row_coef_from<false>(ri, 0.0, pdiff);
+    const double hd = rc.dval * hh;
+    const double e = exp(hd);
+    lds[lane] = rc.kap;
+    lds[64 + lane] = e;
+    lds[128 + lane] = STAGE2 ? phi2_of(hd, e) : phi1_of(hd, e);
+  }
+  __builtin_amdgcn_wave_barrier();
+  const double* __restrict__ xin = STAGE2 ? st.a : st.x;
+  const double* xr = xin + (int64_t)da * 64 * K + k;
+  const bool inj = (op.flags & PRH_F_INJECTION) != 0 && op.src_slot[da] >= 0;
+  const int slot = inj ? op.src_slot[da] : 0;
+  const int J = op.nsrc;
+  const double finj = sch.finj[step];
+  double coef[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) coef[j] = (inj && j < J) ? sch.src_coef[((int64_t)step * J + j) * K + k] : 0.0;
+  // sliding window of kappa * x over bins i-3 .. i+3
+  double xw[7], kw[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    const int j = q - 3;
+    xw[q] = (j >= 0 && j < 64) ? xr[(int64_t)j * K] : 0.0;
+    kw[q] = (j >= 0 && j < 64) ? lds[j] * xw[q] : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < 64; ++i) {
+    double out = acc[i];
+    if (pdiff) {
+      double dx = 0.0;
+#pragma unroll
+      for (int o = -3; o <= 3; ++o) {
+        if (o == 0 || i + o < 0 || i + o >= 64) continue;
+        const double c = op.dcoef[i * 7 + o + 3];
+        if (c != 0.0) dx += c * kw[o + 3];
+      }
+      out = out + dx;
+    }
+    out = out * dl;
+    if (inj) {
+      double bsum = 0.0;
+      for (int j = 0; j < J && j < 4; ++j) {
+        const double g = op.src_grid[(((int64_t)slot * J + j) * 64 + i) * K + k];
+        bsum = (j == 0) ? coef[j] * g : bsum + coef[j] * g;
+      }
+      out = out + finj * bsum;
+    }
+    const int64_t own = ((int64_t)da * 64 + i) * K + k;
+    const double xi = xw[3];
+    const double e = lds[64 + i];
+    const double phi = lds[128 + i];
+    if (!STAGE2) {
+      double a = e * xi;
+      double sc = phi * out;
+      sc = sc * hh;
+      a = a + sc;
+      st.a[own] = a;
+      st.fphi[own] = out;
+    } else {
+      double sc = out - st.fphi[own];
+      sc = sc * hh;
+      sc = sc * phi;
+      st.x[own] = xi + sc;
+    }
+    // slide the window by one bin
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      xw[q] = xw[q + 1];
+      kw[q] = kw[q + 1];
+    }
+    const int jn = i + 4;
+    xw[6] = (jn < 64) ? xr[(int64_t)jn * K] : 0.0;
+    kw[6] = (jn < 64) ? lds[jn] * xw[6] : 0.0;
+  }
+}
+
+template <bool STAGE2>
+__global__ void __launch_bounds__(WTHREADS) etd2_wide(prh_operator op, prh_schedule sch,
+                                                      prh_state st, int step_local, int use_ctl) {
+  __shared__ double lds_all[WWAVES][3 * 64];
+  int step;
+  bool live;
+  wide_step(st, step_local, use_ctl, step, live);
+  if (!live) return;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int K = op.K;
+  int t, g;
+  wide_coords(op.nwtasks, K / 64, t, g);
+  if (t >= op.nwtasks) return;
+  const int* tk = op.wtasks + 5 * t;
+  const int da = tk[0];
+  const int b_begin = tk[1];
+  const int b_end = tk[2];
+  const int slot = tk[3];
+  const int k = g * 64 + lane;
+  const double* __restrict__ xin = STAGE2 ? st.a : st.x;
+  const int w = sch.win[step];
+  const double* __restrict__ mv = op.mvals + (int64_t)w * op.mvals_stride;
+  double acc[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i) acc[i] = 0.0;
+  for (int b = b_begin; b < b_end; ++b) {
+    const int* blk = op.blocks + 4 * b;
+    const int mo = blk[0];
+    const int olo = blk[1];
+    const int ns = blk[2];
+    const double* m = mv + blk[3];
+    const double* xm = xin + (int64_t)mo * 64 * K + k;
+    if (ns == 1 && olo == 0) {
+#pragma unroll
+      for (int i = 0; i < 64; ++i) acc[i] += m[i] * xm[(int64_t)i * K];
+    } else if (ns == 8 && olo == 0) {
+#pragma unroll
+      for (int j = 0; j < 64; ++j) {
+        const double xj = xm[(int64_t)j * K];
+#pragma unroll
+        for (int s2 = 0; s2 < 8; ++s2) {
+          const int i = j - s2;
+          if (i >= 0) acc[i] += m[s2 * 64 + i] * xj;
+        }
+      }
+    } else if (ns == 7 && olo == 1) {
+#pragma unroll
+      for (int j = 0; j < 64; ++j) {
+        const double xj = xm[(int64_t)j * K];
+#pragma unroll
+        for (int s2 = 0; s2 < 7; ++s2) {
+          const int i = j - 1 - s2;
+          if (i >= 0) acc[i] += m[s2 * 64 + i] * xj;
+        }
+      }
+    } else {
+      for (int s2 = 0; s2 < ns; ++s2) {
+        const int off = olo + s2;
+#pragma unroll
+        for (int i = 0; i < 64; ++i) {
+          const int j = i + off;
+          if (j >= 0 && j < 64) acc[i] += m[s2 * 64 + i] * xm[(int64_t)j * K];
+        }
+      }
+    }
+  }
+  if (slot >= 0) {
+#pragma unroll
+    for (int i = 0; i < 64; ++i) st.wpart[((int64_t)slot * 64 + i) * K + k] = acc[i];
+    return;
+  }
+  wide_epilogue<STAGE2>(op, sch, st, step, w, da, k, acc, lds_all[threadIdx.x / WAVE]);
+}
+
+template <bool STAGE2>
+__global__ void __launch_bounds__(WTHREADS) etd2_wide_combine(prh_operator op, prh_schedule sch,
+                                                              prh_state st, int step_local,
+                                                              int use_ctl) {
+  __shared__ double lds_all[WWAVES][3 * 64];
+  int step;
+  bool live;
+  wide_step(st, step_local, use_ctl, step, live);
+  if (!live) return;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int K = op.K;
+  int t, g;
+  wide_coords(op.nheavy, K / 64, t, g);
+  if (t >= op.nheavy) return;
+  const int* hv = op.wheavy + 3 * t;
+  const int da = hv[0];
+  const int slot0 = hv[1];
+  const int np = hv[2];
+  const int k = g * 64 + lane;
+  double acc[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i) acc[i] = st.wpart[((int64_t)slot0 * 64 + i) * K + k];
+  for (int p = 1; p < np; ++p) {
+#pragma unroll
+    for (int i = 0; i < 64; ++i) acc[i] += st.wpart[((int64_t)(slot0 + p) * 64 + i) * K + k];
+  }
+  const int w = sch.win[step];
+  wide_epilogue<STAGE2>(op, sch, st, step, w, da, k, acc, lds_all[threadIdx.x / WAVE]);
+}
+
+template <bool S2>
+void launch_wide(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
+                 int step_local, int use_ctl, hipStream_t s) {
+  const int ng = op.K / 64;
+  const int64_t nwg = (int64_t)((op.nwtasks + WWAVES - 1) / WWAVES) * ng;
+  hipLaunchKernelGGL((etd2_wide<S2>), dim3((unsigned)nwg), dim3(WTHREADS), 0, s, op, sch, st,
+                     step_local, use_ctl);
+  if (op.nheavy > 0) {
+    const int64_t nwh = (int64_t)((op.nheavy + WWAVES - 1) / WWAVES) * ng;
+    hipLaunchKernelGGL((etd2_wide_combine<S2>), dim3((unsigned)nwh), dim3(WTHREADS), 0, s, op,
+                       sch, st, step_local, use_ctl);
+  }
+}
+
 int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_state* st,
                  const char* who) {
   if (!op || !sch || !st) return set_error(std::string(who) + ": null argument");
@@ -641,6 +959,16 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
     return set_error(std::string(who) + ": kt must be 1, 2, 4 or 8");
   if ((int64_t)(op->ntasks / SWG) * ((op->K + op->kt - 1) / op->kt) > 0x7fffffff)
     return set_error(std::string(who) + ": grid too large");
+  if (op->wide) {
+    if (op->K % 64 != 0 || op->d != 64)
+      return set_error(std::string(who) + ": the wide layout needs K % 64 == 0 and d == 64");
+    if (op->mvals_e) return set_error(std::string(who) + ": the wide layout has one photon field");
+    if (!op->wtasks || op->nwtasks <= 0) return set_error(std::string(who) + ": no wide tasks");
+    if (op->nheavy > 0 && (!op->wheavy || !st->wpart))
+      return set_error(std::string(who) + ": split rows need wheavy and the partial buffer");
+    if (op->nsrc > 4) return set_error(std::string(who) + ": the wide layout takes at most 4 sources per rhs");
+    if (st->rec_x) return set_error(std::string(who) + ": the wide layout does not record steps");
+  }
   if ((op->mvals_e == nullptr) != (op->escale == nullptr) ||
       (op->mvals_e != nullptr && op->rpair_e == nullptr))
     return set_error(std::string(who) + ": mvals_e, rpair_e and escale go together");
@@ -675,6 +1003,10 @@ void launch_stage_two(const prh_operator& op, const prh_schedule& sch, const prh
 template <bool S2>
 void launch_stage_kt(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
                      int step_local, int use_ctl, hipStream_t s) {
+  if (op.wide) {
+    launch_wide<S2>(op, sch, st, step_local, use_ctl, s);
+    return;
+  }
   if (op.mvals_e)
     launch_stage_two<S2, true>(op, sch, st, step_local, use_ctl, s);
   else
@@ -777,6 +1109,7 @@ int32_t prh_adjoint_seed(const prh_operator* opT, const prh_schedule* sch,
   if (!adj) return set_error("prh_adjoint_seed: null adjoint");
   prh_state dummy{};
   if (int32_t e = validate(opT, sch, &dummy, "prh_adjoint_seed")) return e;
+  if (opT->wide) return set_error("prh_adjoint_seed: the adjoint runs on the narrow layout (wide = 0)");
   if (step < 0 || step >= sch->nsteps) return set_error("prh_adjoint_seed: step outside the schedule");
   if (!adj->lam || !adj->u) return set_error("prh_adjoint_seed: lam and u are required");
   hipStream_t s = (hipStream_t)stream;
@@ -793,6 +1126,7 @@ int32_t prh_adjoint_step(const prh_operator* opT, const prh_schedule* sch, const
   if (!adj) return set_error("prh_adjoint_step: null adjoint");
   prh_state dummy{};
   if (int32_t e = validate(opT, sch, &dummy, "prh_adjoint_step")) return e;
+  if (opT->wide) return set_error("prh_adjoint_step: the adjoint runs on the narrow layout (wide = 0)");
   if (step < 0 || step >= sch->nsteps) return set_error("prh_adjoint_step: step outside the schedule");
   if (!adj->lam || !adj->lam_a || !adj->u || !adj->v)
     return set_error("prh_adjoint_step: lam, lam_a, u and v are required");

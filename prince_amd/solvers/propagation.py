@@ -35,6 +35,7 @@ from .layout import (
     WindowLayout,
     build_block_layout,
     build_tasks,
+    build_wide_tasks,
     fd_tables,
     fd_transposed_tables,
     transposed_layout,
@@ -238,6 +239,29 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
                 return kt
         return 1
 
+    #: stage-kernel layout: "auto" (wide when K % 64 == 0, d == 64, one photon
+    #: field, at most 4 sources per column and no adjoint), "wide" or "narrow"
+    stage_layout = "auto"
+
+    def wide_part_cost_for(self, K):
+        """Cost units per task of the wide kernel (a block costs 1 + nslot)."""
+        tuned = getattr(self, "_tuned", {}).get(("wide_part_cost", K))
+        return tuned if tuned is not None else 24
+
+    def _use_wide(self, with_adjoint=False):
+        """Whether the solve runs the wide stage kernels (lanes = right-hand sides)."""
+        mode = getattr(self, "_tuned", {}).get("layout", self.stage_layout)
+        K, d = self._K, self.prince_run.cr_grid.d
+        J = max([1] + [len(c) for c in self._column_sources()])
+        ok = (K % 64 == 0 and d == 64 and self._split_fields() is None and not with_adjoint
+              and J <= 4)
+        if mode == "narrow":
+            return False
+        if mode == "wide" and not ok:
+            raise ValueError("the wide stage layout needs K % 64 == 0, d == 64, one photon "
+                             "field, at most 4 sources per column and no adjoint")
+        return ok
+
     def __init__(self, *args, **kwargs):
         super().__init__(*args, **kwargs)
         self._lib = _lib.load()
@@ -263,16 +287,25 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         return np.zeros((self.dim_states, self._K)) if self._K > 1 else np.zeros(self.dim_states)
 
     def _coerce_state(self, state):
+        """Host state (dim_states[, K]) to the device layout: [nspec][d][K] for the
+        wide kernels, [nspec][K][d] otherwise."""
         torch = _torch()
         nspec, d = self.spec_man.nspec, self.prince_run.cr_grid.d
         K = self._K
-        host = np.asarray(state, dtype=np.float64).reshape(nspec, d, K).transpose(0, 2, 1)
+        self._state_wide = self._use_wide()
+        host = np.asarray(state, dtype=np.float64).reshape(nspec, d, K)
+        if not self._state_wide:
+            host = host.transpose(0, 2, 1)
         return torch.from_numpy(np.ascontiguousarray(host)).to(self.device)
+
+    def state_view(self, state):
+        """Device state as an [nspec, d, K] view, whichever layout it is stored in."""
+        return state if getattr(self, "_state_wide", False) else state.permute(0, 2, 1)
 
     def _finalize_state(self, state):
         nspec, d = self.spec_man.nspec, self.prince_run.cr_grid.d
         K = self._K
-        host = state.cpu().numpy().reshape(nspec, K, d).transpose(0, 2, 1).reshape(nspec * d, K)
+        host = self.state_view(state).cpu().numpy().reshape(nspec * d, K)
         return np.ascontiguousarray(host[:, 0] if K == 1 else host)
 
     def close(self):
@@ -455,12 +488,20 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         stride = wl.stride
         stream = torch.cuda.current_stream(dev).cuda_stream
         t1 = time.perf_counter()
-        tasks = build_tasks(wl.fwd, self.part_cost_for(K), _lib.WG_WAVES)
+        wide = self._use_wide(with_adjoint)
+        self._state_wide = wide
+        if wide:
+            tasks = np.zeros((0, 6), dtype=np.int32)
+            wtasks, wheavy, nwslots = build_wide_tasks(wl.fwd, self.wide_part_cost_for(K))
+        else:
+            tasks = build_tasks(wl.fwd, self.part_cost_for(K), _lib.WG_WAVES)
         kt = self.kt_for(K)
         col_sources = self._column_sources()
         has_src = any(len(c) for c in col_sources) and self.enable_injection_jacobian
         h, dldz, hadia, finj, coef, J = self._schedule_arrays(z_grid, col_sources)
         src_slot, src_grid, slots = self._source_tables(col_sources, J)
+        if wide:
+            src_grid = src_grid.transpose(0, 1, 3, 2)  # [slot][J][d][K]
         dcoef, ddiag = self._D_tables
         pscale = np.asarray(self.pair_loss_rates_grid.species_scale, dtype=np.float64)
 
@@ -474,6 +515,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         if has_src:
             flags |= _lib.PRH_F_INJECTION
         blocks = wl.blocks if wl.blocks.size else np.zeros((1, 4))
+        sshape = (nspec, d, K) if wide else (nspec, K, d)
         bufs = {
             "tasks": dv(tasks, np.int32),
             "blocks": dv(blocks, np.int32),
@@ -493,19 +535,24 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             "finj": dv(finj, np.float64),
             "win": dv(win, np.int32),
             "src_coef": dv(coef, np.float64),
-            "a": torch.zeros((nspec, K, d), dtype=torch.float64, device=dev),
-            "fphi": torch.zeros((nspec, K, d), dtype=torch.float64, device=dev),
+            "a": torch.zeros(sshape, dtype=torch.float64, device=dev),
+            "fphi": torch.zeros(sshape, dtype=torch.float64, device=dev),
             "step_ctl": torch.zeros(2, dtype=torch.int32, device=dev),
         }
+        if wide:
+            bufs["wtasks"] = dv(wtasks, np.int32)
+            bufs["wheavy"] = dv(wheavy if wheavy.size else np.zeros((1, 3)), np.int32)
+            bufs["wpart"] = torch.empty((max(1, nwslots), d, K), dtype=torch.float64, device=dev)
         if two:
             bufs["mvals_e"] = torch.empty((nwin, stride), dtype=torch.float64, device=dev)
             bufs["phot_e"] = dv(phot_e, np.float64)
             bufs["rpair_e"] = dv(rpair_e, np.float64)
             bufs["escale"] = dv(self._field_split[1], np.float64)
 
-        def make_op(tk, blk, dcf):
+        def make_op(tk, blk, dcf, wide_op=False):
             return _lib.PrhOperator(
-                nspec=nspec, d=d, K=K, ntasks=tk.shape[0], tasks=tk.data_ptr(),
+                nspec=nspec, d=d, K=K, ntasks=tk.shape[0],
+                tasks=tk.data_ptr() if tk.numel() else None,
                 blocks=blk.data_ptr(), mvals_stride=stride, diag_off=wl.diag_off,
                 mvals=bufs["mvals"].data_ptr(), egrid=bufs["egrid"].data_ptr(),
                 dcoef=dcf.data_ptr(), ddiag=bufs["ddiag"].data_ptr(),
@@ -515,9 +562,14 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
                 mvals_e=bufs["mvals_e"].data_ptr() if two else None,
                 rpair_e=bufs["rpair_e"].data_ptr() if two else None,
                 escale=bufs["escale"].data_ptr() if two else None,
+                wide=1 if wide_op else 0,
+                nwtasks=int(wtasks.shape[0]) if wide_op else 0,
+                wtasks=bufs["wtasks"].data_ptr() if wide_op else None,
+                nheavy=int(wheavy.shape[0]) if wide_op else 0, pad2_=0,
+                wheavy=bufs["wheavy"].data_ptr() if wide_op else None,
             )
 
-        op = make_op(bufs["tasks"], bufs["blocks"], bufs["dcoef"])
+        op = make_op(bufs["tasks"], bufs["blocks"], bufs["dcoef"], wide_op=wide)
         opT = None
         if with_adjoint:
             tasks_t = build_tasks(wl.tr, self.part_cost_for(K), _lib.WG_WAVES)
@@ -530,19 +582,22 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             hadia=bufs["hadia"].data_ptr(), finj=bufs["finj"].data_ptr(),
             win=bufs["win"].data_ptr(), src_coef=bufs["src_coef"].data_ptr(),
         )
-        x = torch.zeros((nspec, K, d), dtype=torch.float64, device=dev)
+        x = torch.zeros(sshape, dtype=torch.float64, device=dev)
         st = _lib.PrhState(
             x=x.data_ptr(), a=bufs["a"].data_ptr(), fphi=bufs["fphi"].data_ptr(),
             step_ctl=bufs["step_ctl"].data_ptr(), rec_x=None, rec_a=None, rec_fx=None,
             rec_fa=None, rec_first=0, rec_count=0,
+            wpart=bufs["wpart"].data_ptr() if wide else None,
         )
         self._plan = {
             "op": op, "opT": opT, "sch": sch, "st": st, "x": x, "bufs": bufs,
             "nsteps": len(h), "layout": wl, "nwin": nwin, "zs": zs, "tasks": tasks, "kt": kt,
             "stream": stream, "stride": stride, "dph": cache.dph, "K": K,
             "fold_rows": stride, "graph": None, "two": two, "z_grid": z_grid,
-            "src_slots": slots, "J": J, "win": win,
+            "src_slots": slots, "J": J, "win": win, "wide": wide,
         }
+        if wide:
+            self._plan.update(wtasks=wtasks, wheavy=wheavy, nwslots=nwslots)
         if getattr(self.backend, "use_hip_graphs", True) and len(h) >= self.graph_chunk:
             self._plan["graph"] = _lib.StepGraph(op, sch, st, self.graph_chunk)
         self.refresh_zs = zs

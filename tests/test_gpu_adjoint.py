@@ -41,27 +41,76 @@ def test_two_field_columns_match_oracle_with_scaled_ebl():
         run.photon_field = base
 
 
+def _fd(fun, theta, p, h):
+    """Richardson-extrapolated central difference (error O(h^4)) along parameter p."""
+
+    def cd(step):
+        tp, tm = theta.copy(), theta.copy()
+        tp[:, p] += step
+        tm[:, p] -= step
+        return (fun(tp) - fun(tm)) / (2 * step)
+
+    return (4 * cd(h / 2) - cd(h)) / 3
+
+
+def _steps(theta, p):
+    # relative step 1e-6 of each physical parameter; R_max itself for its log10
+    if p == 1:
+        return 1e-6 / np.log(10.0) * np.ones(theta.shape[0])
+    return 1e-6 * np.maximum(np.abs(theta[:, p]), 1.0)
+
+
 def test_adjoint_gradient_matches_central_differences():
     prob = AdjointProblem(K=8, z_max=0.25)
     theta = draw_params(8, seed=7)
     losses, grad, tim = prob.loss_and_grad(theta)
     print("losses", losses, "timings", tim)
-    # central differences of the GPU forward with the same knots and loss bins
-    ip, mk = prob.last_loss_plan
+    ip, mk = prob.last_loss_plan  # same knots and loss bins for every evaluation
+
+    def fun(t):
+        _, _, st = prob.forward(t, with_adjoint=False)
+        return prob.loss_of_state(st, ip, mk)[0].detach().cpu().numpy()
+
     worst = 0.0
     for p in range(8):
-        # relative step 1e-6 of each physical parameter: R_max itself for its log10
-        h = 1e-6 / np.log(10.0) * np.ones(8) if p == 1 else 1e-6 * np.maximum(np.abs(theta[:, p]), 1.0)
-        tp, tm = theta.copy(), theta.copy()
-        tp[:, p] += h
-        tm[:, p] -= h
-        _, _, sp = prob.forward(tp, with_adjoint=False)
-        lp = prob.loss_of_state(sp, ip, mk)[0].detach().cpu().numpy()
-        _, _, sm = prob.forward(tm, with_adjoint=False)
-        lm = prob.loss_of_state(sm, ip, mk)[0].detach().cpu().numpy()
-        fd = (lp - lm) / (2 * h)
-        err = np.abs(grad[:, p] - fd) / np.maximum(np.abs(fd), 1e-9 * np.abs(losses) / h + 1e-300)
-        print(f"param {p}: adj {grad[:3, p]} fd {fd[:3]} max rel {err.max():.2e}")
+        fd = _fd(fun, theta, p, _steps(theta, p))
+        err = np.abs(grad[:, p] - fd) / np.abs(fd)
+        print(f"param {p}: adj {grad[:, p]} fd {fd} rel {err}")
         assert np.all(np.isfinite(fd)) and np.all(np.isfinite(grad[:, p]))
         worst = max(worst, float(err.max()))
     assert worst <= 1e-5
+
+
+def test_adjoint_gradient_matches_cpu_oracle_differences():
+    """Adjoint on the GPU vs central differences of the CPU oracle (one instance)."""
+    import torch
+
+    from prince_amd.adjoint import make_source
+
+    prob = AdjointProblem(K=1, z_max=0.06, target_seed=99)
+    theta = draw_params(1, seed=21)
+    _, grad, _ = prob.loss_and_grad(theta)
+    ip, mk = prob.last_loss_plan
+    run = prob.run
+    base = run.photon_field
+    amps = base.model_list[1].amplitudes
+
+    def fun(t):
+        th = t[0]
+        run.photon_field = CombinedPhotonField(
+            [CMBPhotonSpectrum(), SyntheticEBL(amplitudes=amps, scale=th[7])]
+        )
+        try:
+            o = OracleETD2(0.06, 0.0, run)
+            o.add_source_class(make_source(run, th))
+            o.solve(1e-3)
+        finally:
+            run.photon_field = base
+        st = torch.from_numpy(o.state.reshape(run.spec_man.nspec, run.cr_grid.d, 1)).to(mk.device)
+        return prob.loss_of_state(st, ip, mk)[0].detach().cpu().numpy()
+
+    for p in range(8):
+        fd = _fd(fun, theta, p, _steps(theta, p))
+        rel = abs(grad[0, p] - fd[0]) / abs(fd[0])
+        print(f"param {p}: adj {grad[0, p]:.10e} cpu-fd {fd[0]:.10e} rel {rel:.2e}")
+        assert np.isfinite(fd[0]) and rel <= 1e-5
