@@ -99,6 +99,29 @@ typedef struct prh_schedule {
   const double* src_coef; /* [nsteps][J][K] norm * evolution(z) per source */
 } prh_schedule;
 
+/* Co-evolved EM cascade (reference propagation.py:555-571, 620-665, 836-878):
+ * photons on their own grid of d_em <= PRH_EM_MAX bins ride the ETD2 step with
+ * the adiabatic FD operator; after the step the Bethe-Heitler pairs of the
+ * charged nuclei, w[j][k] = bhw[win][j] * sum_s cbh[s] x[s][j][k], are spread
+ * by R_BH into e+ and e- and the window's transfer acts:
+ * gam <- T_gamma gam + T_electron (e- + e+), e+- <- 0 (the e+- blocks are
+ * zero at every step boundary and are not stored). */
+#define PRH_EM_MAX 144
+typedef struct prh_em {
+  int32_t d_em;         /* EM bins */
+  int32_t nwin;         /* windows of bhw, rbh, tg, te */
+  const double* egrid;  /* [d_em] EM bin centres */
+  const double* dcoef;  /* [d_em][7] FD off-diagonal coefficients (offsets -3..3) */
+  const double* ddiag;  /* [d_em] FD diagonal */
+  const double* cbh;    /* [nspec] A_s * pair-loss scale of species s */
+  const double* bhw;    /* [nwin][d] dl_step * (dE/E) * pair rate of the window */
+  const double* rbh;    /* [nwin][d_em][d] Bethe-Heitler e+- shape (one sign) */
+  const double* tg;     /* [nwin][d_em][d_em] photon transfer T_gamma */
+  const double* te;     /* [nwin][d_em][d_em] e+- transfer T_electron */
+  double* gam;          /* [d_em][K] photon state, updated in place */
+  double* wbh;          /* [d][K] scratch: Bethe-Heitler weights of the step */
+} prh_em;
+
 typedef struct prh_state {
   double* x;         /* [nspec][K][d] solution, updated in place */
   double* a;         /* [nspec][K][d] stage-1 vector */
@@ -113,6 +136,7 @@ typedef struct prh_state {
   int32_t rec_first;
   int32_t rec_count;
   double* wpart;     /* [nslots][d][K] partial row sums (wide layout) or NULL */
+  const prh_em* em;  /* host pointer: EM sector stepped after stage 2, or NULL */
 } prh_state;
 
 /* Adjoint sweep buffers (all [nspec][K][d] unless noted). */
@@ -176,6 +200,16 @@ int32_t prh_adjoint_step(const prh_operator* opT, const prh_schedule* sch, const
 /* Adjoint steps k_end-1 down to k_begin (prh_adjoint_step in a loop). */
 int32_t prh_adjoint_steps(const prh_operator* opT, const prh_schedule* sch, const prh_adjoint* adj,
                           int32_t k_begin, int32_t k_end, void* stream);
+/* One EM-sector step (photon ETD2 step, Bethe-Heitler injection, cascade
+ * transfer) after the nuclear step `step`; `st` supplies the post-step nuclear
+ * state.  prh_etd2_steps, prh_etd2_stage (stage 2) and graphs run it by
+ * themselves when st->em is set.  Replaces the step hook of the reference's
+ * solve loop (propagation.py:867-878): _inject_bh_pairs (620-665) followed by
+ * _apply_em_cascade (555-571), with the photon blocks of the ETD2 step
+ * (etd2.py:412-439). */
+int32_t prh_em_step(const prh_operator* op, const prh_schedule* sch, const prh_state* st,
+                    int32_t step, void* stream);
+
 /* u = h phi2 lam for step `step`, before the first prh_adjoint_step of a sweep. */
 int32_t prh_adjoint_seed(const prh_operator* opT, const prh_schedule* sch, const prh_adjoint* adj,
                          int32_t step, void* stream);

@@ -25,6 +25,7 @@ EXPORTED = (
     "prh_adjoint_step",
     "prh_adjoint_seed",
     "prh_adjoint_steps",
+    "prh_em_step",
 )
 
 WG_WAVES = 16
@@ -97,7 +98,28 @@ class PrhState(ctypes.Structure):
         ("rec_first", _i32),
         ("rec_count", _i32),
         ("wpart", _p),
+        ("em", _p),
     ]
+
+
+class PrhEm(ctypes.Structure):
+    _fields_ = [
+        ("d_em", _i32),
+        ("nwin", _i32),
+        ("egrid", _p),
+        ("dcoef", _p),
+        ("ddiag", _p),
+        ("cbh", _p),
+        ("bhw", _p),
+        ("rbh", _p),
+        ("tg", _p),
+        ("te", _p),
+        ("gam", _p),
+        ("wbh", _p),
+    ]
+
+
+EM_MAX = 144
 
 
 class PrhAdjoint(ctypes.Structure):
@@ -136,6 +158,11 @@ def load():
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
             "or `python -m prince_amd.build`; the HIP backend has no fallback."
         )
+    # torch ships its own HIP runtime; loading it first makes this library bind
+    # to that same runtime (one HIP context per process, so the stream handles
+    # and device pointers torch hands over are valid here)
+    import torch  # noqa: F401
+
     lib = ctypes.CDLL(LIB_PATH)
     lib.prh_version.restype = _i32
     lib.prh_last_error.restype = ctypes.c_char_p
@@ -158,6 +185,14 @@ def load():
         ctypes.POINTER(PrhSchedule),
         ctypes.POINTER(PrhState),
         _i32,
+        _i32,
+        _p,
+    ]
+    lib.prh_em_step.restype = _i32
+    lib.prh_em_step.argtypes = [
+        ctypes.POINTER(PrhOperator),
+        ctypes.POINTER(PrhSchedule),
+        ctypes.POINTER(PrhState),
         _i32,
         _p,
     ]

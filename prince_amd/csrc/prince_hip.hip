@@ -739,15 +739,35 @@ __device__ __forceinline__ void wide_coords(int ntask, int ngroup, int& task, in
     g = L / nwg_t;
     tw = L % nwg_t;
   }
-  task = tw * WWAVES + wv;
+  // wave-uniform by construction; readfirstlane lets the compiler keep every
+  // task, block descriptor and coupling value in scalar registers / loads
+  task = __builtin_amdgcn_readfirstlane(tw * WWAVES + wv);
+  g = __builtin_amdgcn_readfirstlane(g);
+}
+
+// Buffer access to one (species, 64-rhs group) tile: element (bin i, lane) at
+// byte offset i * K * 8 (scalar soffset) + lane * 8 (vector voffset) from a
+// uniform base, so no per-lane 64-bit address arithmetic is needed.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const double* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0, bytes, 0x00020000);
+}
+__device__ __forceinline__ double tload(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void tstore(double v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
 }
 
 template <bool STAGE2>
 __device__ __forceinline__ void wide_epilogue(const prh_operator& op, const prh_schedule& sch,
                                               const prh_state& st, int step, int w, int da,
-                                              int k, double (&acc)[64], double* lds) {
+                                              int g, double (&acc)[64], double* lds) {
   const int lane = threadIdx.x & (WAVE - 1);
   const int K = op.K;
+  const int K8 = K * 8;
+  const int v8 = lane * 8;
+  const int k = g * WAVE + lane;
   const bool pdiff = (op.flags & PRH_F_PARTIAL_DIFF) != 0;
   const double hh = sch.h[step];
   const double dl = sch.dldz[step];
@@ -762,21 +782,28 @@ __device__ __forceinline__ void wide_epilogue(const prh_operator& op, const prh_
     lds[128 + lane] = STAGE2 ? phi2_of(hd, e) : phi1_of(hd, e);
   }
   __builtin_amdgcn_wave_barrier();
-  const double* __restrict__ xin = STAGE2 ? st.a : st.x;
-  const double* xr = xin + (int64_t)da * 64 * K + k;
+  const int64_t rowbase = (int64_t)da * 64 * K + (int64_t)g * WAVE;
+  const int tile = 64 * K8;
+  const auto rx = tile_rsrc((STAGE2 ? st.a : st.x) + rowbase, tile);
+  const auto ra = tile_rsrc(st.a + rowbase, tile);
+  const auto rf = tile_rsrc(st.fphi + rowbase, tile);
+  const auto rxo = tile_rsrc(st.x + rowbase, tile);
   const bool inj = (op.flags & PRH_F_INJECTION) != 0 && op.src_slot[da] >= 0;
   const int slot = inj ? op.src_slot[da] : 0;
-  const int J = op.nsrc;
+  const int J = op.nsrc < 4 ? op.nsrc : 4;
   const double finj = sch.finj[step];
   double coef[4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) coef[j] = (inj && j < J) ? sch.src_coef[((int64_t)step * J + j) * K + k] : 0.0;
-  // sliding window of kappa * x over bins i-3 .. i+3
+  for (int j = 0; j < 4; ++j)
+    coef[j] = (inj && j < J) ? sch.src_coef[((int64_t)step * op.nsrc + j) * K + k] : 0.0;
+  const auto rs = tile_rsrc(op.src_grid + (int64_t)slot * op.nsrc * 64 * K + (int64_t)g * WAVE,
+                            (J > 0 ? J : 1) * tile);
+  // sliding window of x and kappa * x over bins i-3 .. i+3
   double xw[7], kw[7];
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
     const int j = q - 3;
-    xw[q] = (j >= 0 && j < 64) ? xr[(int64_t)j * K] : 0.0;
+    xw[q] = (j >= 0 && j < 64) ? tload(rx, v8, j * K8) : 0.0;
     kw[q] = (j >= 0 && j < 64) ? lds[j] * xw[q] : 0.0;
   }
 #pragma unroll
@@ -795,13 +822,12 @@ __device__ __forceinline__ void wide_epilogue(const prh_operator& op, const prh_
     out = out * dl;
     if (inj) {
       double bsum = 0.0;
-      for (int j = 0; j < J && j < 4; ++j) {
-        const double g = op.src_grid[(((int64_t)slot * J + j) * 64 + i) * K + k];
-        bsum = (j == 0) ? coef[j] * g : bsum + coef[j] * g;
+      for (int j = 0; j < J; ++j) {
+        const double gv = tload(rs, v8, (j * 64 + i) * K8);
+        bsum = (j == 0) ? coef[j] * gv : bsum + coef[j] * gv;
       }
       out = out + finj * bsum;
     }
-    const int64_t own = ((int64_t)da * 64 + i) * K + k;
     const double xi = xw[3];
     const double e = lds[64 + i];
     const double phi = lds[128 + i];
@@ -810,13 +836,13 @@ __device__ __forceinline__ void wide_epilogue(const prh_operator& op, const prh_
       double sc = phi * out;
       sc = sc * hh;
       a = a + sc;
-      st.a[own] = a;
-      st.fphi[own] = out;
+      tstore(a, ra, v8, i * K8);
+      tstore(out, rf, v8, i * K8);
     } else {
-      double sc = out - st.fphi[own];
+      double sc = out - tload(rf, v8, i * K8);
       sc = sc * hh;
       sc = sc * phi;
-      st.x[own] = xi + sc;
+      tstore(xi + sc, rxo, v8, i * K8);
     }
     // slide the window by one bin
 #pragma unroll
@@ -825,8 +851,52 @@ __device__ __forceinline__ void wide_epilogue(const prh_operator& op, const prh_
       kw[q] = kw[q + 1];
     }
     const int jn = i + 4;
-    xw[6] = (jn < 64) ? xr[(int64_t)jn * K] : 0.0;
+    xw[6] = (jn < 64) ? tload(rx, v8, jn * K8) : 0.0;
     kw[6] = (jn < 64) ? lds[jn] * xw[6] : 0.0;
+  }
+}
+
+// Band block of NS diagonals starting at offset OLO:
+// acc[i] += sum_{s < NS} m[s * 64 + i] * x[i + OLO + s], each x row loaded once.
+template <int OLO, int NS>
+__device__ __forceinline__ void wide_band(__amdgpu_buffer_rsrc_t r, const double* __restrict__ m,
+                                          int K8, int v8, double (&acc)[64]) {
+  constexpr int JEND = 64 - OLO;  // rows jx = jj + OLO < 64
+#pragma unroll
+  for (int j0 = 0; j0 < JEND; j0 += 8) {
+    double xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      xv[u] = (j0 + u < JEND) ? tload(r, v8, (j0 + u + OLO) * K8) : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int jj = j0 + u;
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) {
+        const int i = jj - s2;
+        if (jj < JEND && i >= 0 && i < 64) acc[i] += m[s2 * 64 + i] * xv[u];
+      }
+    }
+  }
+}
+
+// One block into acc.  The host admits only diagonal (0, 1) and the band
+// shapes (0, 8) and (1, 7) to this layout.
+__device__ __forceinline__ void wide_block(__amdgpu_buffer_rsrc_t r, const double* __restrict__ m,
+                                           int olo, int ns, int K8, int v8, double (&acc)[64]) {
+  if (ns == 1) {
+#pragma unroll
+    for (int i0 = 0; i0 < 64; i0 += 16) {
+      double xv[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) xv[u] = tload(r, v8, (i0 + u) * K8);
+#pragma unroll
+      for (int u = 0; u < 16; ++u) acc[i0 + u] += m[i0 + u] * xv[u];
+    }
+  } else if (olo == 0) {
+    wide_band<0, 8>(r, m, K8, v8, acc);
+  } else {
+    wide_band<1, 7>(r, m, K8, v8, acc);
   }
 }
 
@@ -848,60 +918,26 @@ __global__ void __launch_bounds__(WTHREADS) etd2_wide(prh_operator op, prh_sched
   const int b_begin = tk[1];
   const int b_end = tk[2];
   const int slot = tk[3];
-  const int k = g * 64 + lane;
-  const double* __restrict__ xin = STAGE2 ? st.a : st.x;
+  const double* __restrict__ xg = (STAGE2 ? st.a : st.x) + (int64_t)g * WAVE;
   const int w = sch.win[step];
   const double* __restrict__ mv = op.mvals + (int64_t)w * op.mvals_stride;
   double acc[64];
 #pragma unroll
   for (int i = 0; i < 64; ++i) acc[i] = 0.0;
+  const int K8 = K * 8;
+  const int v8 = lane * 8;
   for (int b = b_begin; b < b_end; ++b) {
     const int* blk = op.blocks + 4 * b;
-    const int mo = blk[0];
-    const int olo = blk[1];
-    const int ns = blk[2];
-    const double* m = mv + blk[3];
-    const double* xm = xin + (int64_t)mo * 64 * K + k;
-    if (ns == 1 && olo == 0) {
-#pragma unroll
-      for (int i = 0; i < 64; ++i) acc[i] += m[i] * xm[(int64_t)i * K];
-    } else if (ns == 8 && olo == 0) {
-#pragma unroll
-      for (int j = 0; j < 64; ++j) {
-        const double xj = xm[(int64_t)j * K];
-#pragma unroll
-        for (int s2 = 0; s2 < 8; ++s2) {
-          const int i = j - s2;
-          if (i >= 0) acc[i] += m[s2 * 64 + i] * xj;
-        }
-      }
-    } else if (ns == 7 && olo == 1) {
-#pragma unroll
-      for (int j = 0; j < 64; ++j) {
-        const double xj = xm[(int64_t)j * K];
-#pragma unroll
-        for (int s2 = 0; s2 < 7; ++s2) {
-          const int i = j - 1 - s2;
-          if (i >= 0) acc[i] += m[s2 * 64 + i] * xj;
-        }
-      }
-    } else {
-      for (int s2 = 0; s2 < ns; ++s2) {
-        const int off = olo + s2;
-#pragma unroll
-        for (int i = 0; i < 64; ++i) {
-          const int j = i + off;
-          if (j >= 0 && j < 64) acc[i] += m[s2 * 64 + i] * xm[(int64_t)j * K];
-        }
-      }
-    }
+    const auto r = tile_rsrc(xg + (int64_t)blk[0] * 64 * K, 64 * K8);
+    wide_block(r, mv + blk[3], blk[1], blk[2], K8, v8, acc);
   }
   if (slot >= 0) {
+    const auto rp = tile_rsrc(st.wpart + (int64_t)slot * 64 * K + (int64_t)g * WAVE, 64 * K8);
 #pragma unroll
-    for (int i = 0; i < 64; ++i) st.wpart[((int64_t)slot * 64 + i) * K + k] = acc[i];
+    for (int i = 0; i < 64; ++i) tstore(acc[i], rp, v8, i * K8);
     return;
   }
-  wide_epilogue<STAGE2>(op, sch, st, step, w, da, k, acc, lds_all[threadIdx.x / WAVE]);
+  wide_epilogue<STAGE2>(op, sch, st, step, w, da, g, acc, lds_all[threadIdx.x / WAVE]);
 }
 
 template <bool STAGE2>
@@ -922,16 +958,22 @@ __global__ void __launch_bounds__(WTHREADS) etd2_wide_combine(prh_operator op, p
   const int da = hv[0];
   const int slot0 = hv[1];
   const int np = hv[2];
-  const int k = g * 64 + lane;
+  const int K8 = K * 8;
+  const int v8 = lane * 8;
+  const double* pp = st.wpart + (int64_t)slot0 * 64 * K + (int64_t)g * WAVE;
   double acc[64];
+  {
+    const auto r0 = tile_rsrc(pp, 64 * K8);
 #pragma unroll
-  for (int i = 0; i < 64; ++i) acc[i] = st.wpart[((int64_t)slot0 * 64 + i) * K + k];
+    for (int i = 0; i < 64; ++i) acc[i] = tload(r0, v8, i * K8);
+  }
   for (int p = 1; p < np; ++p) {
+    const auto rq = tile_rsrc(pp + (int64_t)p * 64 * K, 64 * K8);
 #pragma unroll
-    for (int i = 0; i < 64; ++i) acc[i] += st.wpart[((int64_t)(slot0 + p) * 64 + i) * K + k];
+    for (int i = 0; i < 64; ++i) acc[i] += tload(rq, v8, i * K8);
   }
   const int w = sch.win[step];
-  wide_epilogue<STAGE2>(op, sch, st, step, w, da, k, acc, lds_all[threadIdx.x / WAVE]);
+  wide_epilogue<STAGE2>(op, sch, st, step, w, da, g, acc, lds_all[threadIdx.x / WAVE]);
 }
 
 template <bool S2>
@@ -948,6 +990,210 @@ void launch_wide(const prh_operator& op, const prh_schedule& sch, const prh_stat
   }
 }
 
+// ---------------------------------------------------------------------------
+// EM sector (photons, e+-): one launch pair per step after stage 2.
+// em_bh_weights: w[j][k] = bhw[win][j] * sum_s cbh[s] x[s][j][k]; one wave per
+// (bin j, 64-rhs group), species summed in state order.
+// em_step: one 1024-thread workgroup per 64-rhs group (lane = rhs); the photon
+// column lives in LDS ([bin][lane], conflict-free); wave v owns bins v + 16 t.
+// ---------------------------------------------------------------------------
+constexpr int EM_WAVES = 16;
+constexpr int EM_THREADS = EM_WAVES * WAVE;
+constexpr int EM_NB = (PRH_EM_MAX + EM_WAVES - 1) / EM_WAVES;
+
+__global__ void __launch_bounds__(256) em_bh_weights(prh_operator op, prh_schedule sch,
+                                                     prh_state st, prh_em em, int64_t sS,
+                                                     int64_t sJ, int64_t sK, int step_local,
+                                                     int use_ctl) {
+  int step;
+  bool live;
+  wide_step(st, step_local, use_ctl, step, live);
+  if (!live) return;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int K = op.K;
+  const int d = op.d;
+  const int ng = (K + WAVE - 1) / WAVE;
+  const int item = blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;
+  if (item >= d * ng) return;
+  const int j = item / ng;
+  const int k = (item % ng) * WAVE + lane;
+  if (k >= K) return;
+  const double* xp = st.x + j * sJ + k * sK;
+  double acc = 0.0;
+  int s = 0;
+  const int ns = op.nspec;
+  for (; s + 4 <= ns; s += 4) {
+    const double x0 = xp[(int64_t)s * sS];
+    const double x1 = xp[(int64_t)(s + 1) * sS];
+    const double x2 = xp[(int64_t)(s + 2) * sS];
+    const double x3 = xp[(int64_t)(s + 3) * sS];
+    acc += em.cbh[s] * x0;
+    acc += em.cbh[s + 1] * x1;
+    acc += em.cbh[s + 2] * x2;
+    acc += em.cbh[s + 3] * x3;
+  }
+  for (; s < ns; ++s) acc += em.cbh[s] * xp[(int64_t)s * sS];
+  const int w = sch.win[step];
+  em.wbh[(int64_t)j * K + k] = em.bhw[(int64_t)w * d + j] * acc;
+}
+
+__global__ void __launch_bounds__(EM_THREADS) em_step_kernel(prh_operator op, prh_schedule sch,
+                                                            prh_state st, prh_em em,
+                                                            int step_local, int use_ctl) {
+  __shared__ double g0[PRH_EM_MAX * WAVE];
+  __shared__ double ga[PRH_EM_MAX * WAVE];
+  int step;
+  bool live;
+  wide_step(st, step_local, use_ctl, step, live);
+  if (!live) return;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = threadIdx.x / WAVE;
+  const int K = op.K;
+  const int d = op.d;
+  const int de = em.d_em;
+  const int k = blockIdx.x * WAVE + lane;
+  const bool act = k < K;
+  const int w = sch.win[step];
+  const double hh = sch.h[step];
+  const double dl = sch.dldz[step];
+  const double had = sch.hadia[step];
+  const bool pdiff = (op.flags & PRH_F_PARTIAL_DIFF) != 0;
+  for (int i = wv; i < de; i += EM_WAVES) g0[i * WAVE + lane] = act ? em.gam[(int64_t)i * K + k] : 0.0;
+  __syncthreads();
+
+  // F(y)_i = dl * sum_{o != 0} D[i][i+o] kappa_{i+o} y_{i+o},  kappa = H cm2sec E
+  auto stencil = [&](const double* y, int i) {
+    double f = 0.0;
+    if (pdiff) {
+#pragma unroll
+      for (int o = -3; o <= 3; ++o) {
+        const int j = i + o;
+        if (o == 0 || j < 0 || j >= de) continue;
+        const double c = em.dcoef[i * 7 + o + 3];
+        if (c != 0.0) f += c * ((had * em.egrid[j]) * y[j * WAVE + lane]);
+      }
+    }
+    return f * dl;
+  };
+  double f0[EM_NB], e_[EM_NB], p1[EM_NB], p2[EM_NB];
+#pragma unroll
+  for (int t = 0; t < EM_NB; ++t) {
+    const int i = wv + EM_WAVES * t;
+    f0[t] = e_[t] = p1[t] = p2[t] = 0.0;
+    if (i >= de) continue;
+    double dv = 0.0;
+    if (pdiff) {
+      double kx = (had * em.egrid[i]) * em.ddiag[i];
+      dv = kx * dl;
+    }
+    const double hd = dv * hh;
+    const double e = exp(hd);
+    e_[t] = e;
+    p1[t] = phi1_of(hd, e);
+    p2[t] = phi2_of(hd, e);
+    const double f = stencil(g0, i);
+    f0[t] = f;
+    double a = e * g0[i * WAVE + lane];
+    double sc = p1[t] * f;
+    sc = sc * hh;
+    ga[i * WAVE + lane] = a + sc;
+  }
+  __syncthreads();
+  double xn[EM_NB];
+#pragma unroll
+  for (int t = 0; t < EM_NB; ++t) {
+    const int i = wv + EM_WAVES * t;
+    xn[t] = 0.0;
+    if (i >= de) continue;
+    const double fa = stencil(ga, i);
+    double sc = fa - f0[t];
+    sc = sc * hh;
+    sc = sc * p2[t];
+    xn[t] = ga[i * WAVE + lane] + sc;
+  }
+  // Bethe-Heitler pairs: e_src = R_BH w, lep = e- + e+ = 2 e_src
+  double es[EM_NB];
+#pragma unroll
+  for (int t = 0; t < EM_NB; ++t) es[t] = 0.0;
+  const double* R = em.rbh + (int64_t)w * de * d;
+  for (int j = 0; j < d; ++j) {
+    const double wj = act ? em.wbh[(int64_t)j * K + k] : 0.0;
+#pragma unroll
+    for (int t = 0; t < EM_NB; ++t) {
+      const int i = wv + EM_WAVES * t;
+      if (i < de) es[t] += R[(int64_t)i * d + j] * wj;
+    }
+  }
+  __syncthreads();  // every wave is done reading g0 and ga
+#pragma unroll
+  for (int t = 0; t < EM_NB; ++t) {
+    const int i = wv + EM_WAVES * t;
+    if (i >= de) continue;
+    g0[i * WAVE + lane] = xn[t];
+    ga[i * WAVE + lane] = es[t] + es[t];
+  }
+  __syncthreads();
+  // cascade transfer: gam <- T_gamma gam + T_electron lep
+  const double* Tg = em.tg + (int64_t)w * de * de;
+  const double* Te = em.te + (int64_t)w * de * de;
+  double ag[EM_NB], ae[EM_NB];
+#pragma unroll
+  for (int t = 0; t < EM_NB; ++t) ag[t] = ae[t] = 0.0;
+  for (int m = 0; m < de; ++m) {
+    const double gm = g0[m * WAVE + lane];
+    const double lm = ga[m * WAVE + lane];
+#pragma unroll
+    for (int t = 0; t < EM_NB; ++t) {
+      const int i = wv + EM_WAVES * t;
+      if (i < de) {
+        ag[t] += Tg[(int64_t)i * de + m] * gm;
+        ae[t] += Te[(int64_t)i * de + m] * lm;
+      }
+    }
+  }
+  if (!act) return;
+#pragma unroll
+  for (int t = 0; t < EM_NB; ++t) {
+    const int i = wv + EM_WAVES * t;
+    if (i < de) em.gam[(int64_t)i * K + k] = ag[t] + ae[t];
+  }
+}
+
+void launch_em(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
+               int step_local, int use_ctl, hipStream_t s) {
+  const prh_em& em = *st.em;
+  const int64_t K = op.K, d = op.d;
+  int64_t sS, sJ, sK;
+  if (op.wide) {
+    sS = d * K;
+    sJ = K;
+    sK = 1;
+  } else {
+    sS = K * d;
+    sJ = 1;
+    sK = d;
+  }
+  const int ng = (op.K + WAVE - 1) / WAVE;
+  const int nitems = op.d * ng;
+  hipLaunchKernelGGL(em_bh_weights, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, s, op, sch,
+                     st, em, sS, sJ, sK, step_local, use_ctl);
+  hipLaunchKernelGGL(em_step_kernel, dim3((unsigned)ng), dim3(EM_THREADS), 0, s, op, sch, st, em,
+                     step_local, use_ctl);
+}
+
+int32_t validate_em(const prh_operator* op, const prh_state* st, const char* who) {
+  if (!st->em) return 0;
+  const prh_em* em = st->em;
+  if (em->d_em < 1 || em->d_em > PRH_EM_MAX)
+    return set_error(std::string(who) + ": d_em must be in [1, 144]");
+  if (!em->egrid || !em->dcoef || !em->ddiag || !em->cbh || !em->bhw || !em->rbh || !em->tg ||
+      !em->te || !em->gam || !em->wbh)
+    return set_error(std::string(who) + ": EM sector with a null buffer");
+  if (em->nwin < 1) return set_error(std::string(who) + ": EM sector without windows");
+  (void)op;
+  return 0;
+}
+
 int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_state* st,
                  const char* who) {
   if (!op || !sch || !st) return set_error(std::string(who) + ": null argument");
@@ -955,6 +1201,8 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
   if (op->K <= 0) return set_error(std::string(who) + ": K must be >= 1");
   if (op->ntasks % SWG != 0)
     return set_error(std::string(who) + ": ntasks must be a multiple of 16");
+  if ((op->wide ? op->nwtasks : op->ntasks) <= 0)
+    return set_error(std::string(who) + ": empty operator");
   if (!(op->kt == 1 || op->kt == 2 || op->kt == 4 || op->kt == 8))
     return set_error(std::string(who) + ": kt must be 1, 2, 4 or 8");
   if ((int64_t)(op->ntasks / SWG) * ((op->K + op->kt - 1) / op->kt) > 0x7fffffff)
@@ -962,8 +1210,10 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
   if (op->wide) {
     if (op->K % 64 != 0 || op->d != 64)
       return set_error(std::string(who) + ": the wide layout needs K % 64 == 0 and d == 64");
+    if ((int64_t)op->K * 64 * 8 * (op->nsrc > 1 ? op->nsrc : 1) >= ((int64_t)1 << 31))
+      return set_error(std::string(who) + ": K too large for the wide layout's 32-bit tile offsets");
     if (op->mvals_e) return set_error(std::string(who) + ": the wide layout has one photon field");
-    if (!op->wtasks || op->nwtasks <= 0) return set_error(std::string(who) + ": no wide tasks");
+    if (!op->wtasks) return set_error(std::string(who) + ": no wide tasks");
     if (op->nheavy > 0 && (!op->wheavy || !st->wpart))
       return set_error(std::string(who) + ": split rows need wheavy and the partial buffer");
     if (op->nsrc > 4) return set_error(std::string(who) + ": the wide layout takes at most 4 sources per rhs");
@@ -972,7 +1222,7 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
   if ((op->mvals_e == nullptr) != (op->escale == nullptr) ||
       (op->mvals_e != nullptr && op->rpair_e == nullptr))
     return set_error(std::string(who) + ": mvals_e, rpair_e and escale go together");
-  return 0;
+  return validate_em(op, st, who);
 }
 
 template <bool S2, bool TWO>
@@ -1046,7 +1296,7 @@ struct prh_graph {
 
 extern "C" {
 
-int32_t prh_version(void) { return 3; }
+int32_t prh_version(void) { return 4; }
 
 const char* prh_last_error(void) { return g_last_error.c_str(); }
 
@@ -1081,11 +1331,12 @@ int32_t prh_etd2_steps(const prh_operator* op, const prh_schedule* sch, const pr
   if (int32_t e = validate(op, sch, st, "prh_etd2_steps")) return e;
   if (k_begin < 0 || k_end > sch->nsteps || k_begin > k_end)
     return set_error("prh_etd2_steps: step range outside the schedule");
-  if (op->ntasks <= 0 || k_begin == k_end) return 0;
+  if (k_begin == k_end) return 0;
   hipStream_t s = (hipStream_t)stream;
   for (int k = k_begin; k < k_end; ++k) {
     launch_stage_kt<false>(*op, *sch, *st, k, 0, s);
     launch_stage_kt<true>(*op, *sch, *st, k, 0, s);
+    if (st->em) launch_em(*op, *sch, *st, k, 0, s);
   }
   return check_hip(hipGetLastError(), "prh_etd2_steps launch");
 }
@@ -1095,13 +1346,23 @@ int32_t prh_etd2_stage(const prh_operator* op, const prh_schedule* sch, const pr
   if (int32_t e = validate(op, sch, st, "prh_etd2_stage")) return e;
   if (step < 0 || step >= sch->nsteps) return set_error("prh_etd2_stage: step outside the schedule");
   if (stage != 1 && stage != 2) return set_error("prh_etd2_stage: stage must be 1 or 2");
-  if (op->ntasks <= 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  if (stage == 1)
+  if (stage == 1) {
     launch_stage_kt<false>(*op, *sch, *st, step, 0, s);
-  else
+  } else {
     launch_stage_kt<true>(*op, *sch, *st, step, 0, s);
+    if (st->em) launch_em(*op, *sch, *st, step, 0, s);
+  }
   return check_hip(hipGetLastError(), "prh_etd2_stage launch");
+}
+
+int32_t prh_em_step(const prh_operator* op, const prh_schedule* sch, const prh_state* st,
+                    int32_t step, void* stream) {
+  if (int32_t e = validate(op, sch, st, "prh_em_step")) return e;
+  if (!st->em) return set_error("prh_em_step: the state has no EM sector");
+  if (step < 0 || step >= sch->nsteps) return set_error("prh_em_step: step outside the schedule");
+  launch_em(*op, *sch, *st, step, 0, (hipStream_t)stream);
+  return check_hip(hipGetLastError(), "prh_em_step launch");
 }
 
 int32_t prh_adjoint_seed(const prh_operator* opT, const prh_schedule* sch,
@@ -1166,7 +1427,6 @@ int32_t prh_graph_create(const prh_operator* op, const prh_schedule* sch, const 
   if (!out) return set_error("prh_graph_create: null output");
   if (!st->step_ctl) return set_error("prh_graph_create: state has no step_ctl buffer");
   if (chunk < 1 || chunk > 4096) return set_error("prh_graph_create: chunk must be in [1, 4096]");
-  if (op->ntasks <= 0) return set_error("prh_graph_create: empty operator");
   *out = nullptr;
   hipStream_t cap = nullptr;
   if (int32_t e = check_hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking),
@@ -1181,6 +1441,7 @@ int32_t prh_graph_create(const prh_operator* op, const prh_schedule* sch, const 
     for (int j = 0; j < chunk; ++j) {
       launch_stage_kt<false>(*op, *sch, *st, j, 1, cap);
       launch_stage_kt<true>(*op, *sch, *st, j, 1, cap);
+      if (st->em) launch_em(*op, *sch, *st, j, 1, cap);
     }
     hipLaunchKernelGGL(step_ctl_advance, dim3(1), dim3(1), 0, cap, st->step_ctl, chunk);
     err = hipStreamEndCapture(cap, &g->graph);

@@ -9,7 +9,7 @@ interaction rates.  The cross sections default to the synthetic model of
 
 from __future__ import annotations
 
-from . import config
+from . import cascade, config
 from .interaction_rates import (
     ContinuousAdiabaticLossRate,
     ContinuousPairProductionLossRate,
@@ -18,6 +18,10 @@ from .interaction_rates import (
 from .photonfields import CMBPhotonSpectrum
 from .units import EnergyGrid, SpeciesManager, get_AZN
 from .xsec import SyntheticPhotoNuclear, nuclear_species
+
+
+#: EM species in PDG order: e+, e-, photon
+EM_PDGS = (-11, 11, 22)
 
 
 class PriNCeRun:
@@ -30,6 +34,7 @@ class PriNCeRun:
         cosmic_ray_grid=None,
         photon_grid=None,
         backend=None,
+        em_cascade=False,
     ):
         self.backend = backend if backend is not None else config.BackendConfig()
         self.cr_grid = EnergyGrid(*(cosmic_ray_grid or config.cosmic_ray_grid))
@@ -50,8 +55,25 @@ class PriNCeRun:
         self.pair_loss_rates_bins = ContinuousPairProductionLossRate(self, energy="bins")
         self.int_rates = PhotoNuclearInteractionRate(self)
         self.em_int_rates = None
-        self.enable_em_cascade = False
         self.spec_man.add_grid("ph", self.ph_grid.d)
+        # co-evolved EM cascade (propagation.py:250-316): photons and e+- on
+        # their own 136-bin grid, driven by Bethe-Heitler pairs of the nuclei
+        self.enable_em_cascade = bool(em_cascade)
+        self.em_grid = EnergyGrid(*cascade.EM_GRID) if em_cascade else None
+        self.em_pdgs = EM_PDGS if em_cascade else ()
+
+    @property
+    def dim_em(self):
+        """Length of the EM part of the state (e+, e-, photon blocks)."""
+        return len(self.em_pdgs) * (self.em_grid.d if self.em_grid is not None else 0)
+
+    def em_slices(self):
+        """{pdg: slice} of the EM blocks, placed after the nuclear states."""
+        if not self.enable_em_cascade:
+            return {}
+        d = self.em_grid.d
+        return {p: slice(self.dim_states + i * d, self.dim_states + (i + 1) * d)
+                for i, p in enumerate(self.em_pdgs)}
 
     def set_photon_field(self, pfield):
         self.photon_field = pfield

@@ -199,6 +199,18 @@ def wide_block_cost(blocks):
     return 1 + blocks[:, 2].astype(np.int64)
 
 
+#: block shapes (o_lo, nslot) the wide stage kernel has code paths for
+WIDE_SHAPES = ((0, 1), (0, 8), (1, 7))
+
+
+def wide_shapes_ok(blocks):
+    """True when every block is one of WIDE_SHAPES."""
+    if blocks.size == 0:
+        return True
+    shp = blocks[:, 1].astype(np.int64) * 16 + blocks[:, 2].astype(np.int64)
+    return bool(np.isin(shp, [o * 16 + n for o, n in WIDE_SHAPES]).all())
+
+
 def build_wide_tasks(layout: BlockLayout, part_cost=24):
     """Tasks of the wide stage kernel: one wave per (task, 64-rhs group).
 

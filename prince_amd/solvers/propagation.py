@@ -36,6 +36,7 @@ from .layout import (
     build_block_layout,
     build_tasks,
     build_wide_tasks,
+    wide_shapes_ok,
     fd_tables,
     fd_transposed_tables,
     transposed_layout,
@@ -129,7 +130,12 @@ class UHECRPropagationSolver:
         self.adia_loss_rates_grid = prince_run.adia_loss_rates_grid
         self.pair_loss_rates_grid = prince_run.pair_loss_rates_grid
         self.dim_states = prince_run.dim_states
-        self.state = np.zeros(prince_run.dim_states)
+        # co-evolved EM cascade (propagation.py:250-316): photon and e+- blocks
+        # follow the nuclear states in ``state``
+        self.enable_em_cascade = bool(getattr(prince_run, "enable_em_cascade", False))
+        self.dim_em = prince_run.dim_em if self.enable_em_cascade else 0
+        self._em_dz = 1e-2
+        self.state = np.zeros(prince_run.dim_states + self.dim_em)
         self.result = None
         self.list_of_sources = []
         self.diff_operator = DifferentialOperator(prince_run.cr_grid, prince_run.spec_man.nspec)
@@ -138,10 +144,16 @@ class UHECRPropagationSolver:
     def known_species(self):
         return self.spec_man.known_species
 
+    def _em_info(self):
+        """{pdg: (grid, slice)} of the EM blocks of ``state`` (empty without EM)."""
+        run = self.prince_run
+        return {p: (run.em_grid.grid, sl) for p, sl in run.em_slices().items()}
+
     @property
     def res(self):
         if self.result is None:
-            self.result = UHECRPropagationResult(self.state, self.egrid, self.spec_man)
+            self.result = UHECRPropagationResult(self.state, self.egrid, self.spec_man,
+                                                 em=self._em_info())
         return self.result
 
     def add_source_class(self, source_instance):
@@ -181,6 +193,7 @@ class UHECRPropagationSolverETD2(UHECRPropagationSolver):
         """Integrate from initial_z down to final_z with steps of |dz|."""
         start = time.time()
         z_grid = z_grid_for(self.initial_z, self.final_z, dz)
+        self._em_dz = float(abs(dz))
         self.current_z_rates = None
         self._reset_for_solve()
         self._ensure_D_split()
@@ -239,14 +252,16 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
                 return kt
         return 1
 
-    #: stage-kernel layout: "auto" (wide when K % 64 == 0, d == 64, one photon
-    #: field, at most 4 sources per column and no adjoint), "wide" or "narrow"
-    stage_layout = "auto"
+    #: stage-kernel layout: "narrow" (lanes = bins, the faster one measured at
+    #: K = 512: 250 vs 360 us per stage), "wide" (lanes = right-hand sides;
+    #: needs K % 64 == 0, d == 64, one photon field, <= 4 sources per column,
+    #: no adjoint) or "auto" (wide whenever those conditions hold)
+    stage_layout = "narrow"
 
     def wide_part_cost_for(self, K):
         """Cost units per task of the wide kernel (a block costs 1 + nslot)."""
         tuned = getattr(self, "_tuned", {}).get(("wide_part_cost", K))
-        return tuned if tuned is not None else 24
+        return tuned if tuned is not None else 96
 
     def _use_wide(self, with_adjoint=False):
         """Whether the solve runs the wide stage kernels (lanes = right-hand sides)."""
@@ -284,7 +299,8 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             self._D_tables = (dcoef, ddiag)
 
     def _init_state(self):
-        return np.zeros((self.dim_states, self._K)) if self._K > 1 else np.zeros(self.dim_states)
+        n = self.dim_states + self.dim_em
+        return np.zeros((n, self._K)) if self._K > 1 else np.zeros(n)
 
     def _coerce_state(self, state):
         """Host state (dim_states[, K]) to the device layout: [nspec][d][K] for the
@@ -293,7 +309,9 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         nspec, d = self.spec_man.nspec, self.prince_run.cr_grid.d
         K = self._K
         self._state_wide = self._use_wide()
-        host = np.asarray(state, dtype=np.float64).reshape(nspec, d, K)
+        state = np.asarray(state, dtype=np.float64).reshape(-1, K)
+        self._em_init = state[self.dim_states:] if self.dim_em else None
+        host = state[: self.dim_states].reshape(nspec, d, K)
         if not self._state_wide:
             host = host.transpose(0, 2, 1)
         return torch.from_numpy(np.ascontiguousarray(host)).to(self.device)
@@ -306,7 +324,16 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         nspec, d = self.spec_man.nspec, self.prince_run.cr_grid.d
         K = self._K
         host = self.state_view(state).cpu().numpy().reshape(nspec * d, K)
+        if self.dim_em:
+            host = np.concatenate([host, self.em_host_state()], axis=0)
         return np.ascontiguousarray(host[:, 0] if K == 1 else host)
+
+    def em_host_state(self, plan=None):
+        """EM blocks (e+, e-, photon) of the last solve as a host array [3 d_em, K]."""
+        plan = plan if plan is not None else self._plan
+        gam = plan["em_bufs"]["gam"].cpu().numpy()
+        z = np.zeros_like(gam)
+        return np.concatenate([z, z, gam], axis=0)
 
     def close(self):
         if self._plan is not None and self._plan.get("graph") is not None:
@@ -488,7 +515,13 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         stride = wl.stride
         stream = torch.cuda.current_stream(dev).cuda_stream
         t1 = time.perf_counter()
+        if with_adjoint and self.enable_em_cascade:
+            raise NotImplementedError("the adjoint sweep does not cover the EM cascade")
         wide = self._use_wide(with_adjoint)
+        if wide and not wide_shapes_ok(wl.fwd.blocks):
+            if getattr(self, "_tuned", {}).get("layout", self.stage_layout) == "wide":
+                raise ValueError("the operator has block shapes the wide stage kernel lacks")
+            wide = False
         self._state_wide = wide
         if wide:
             tasks = np.zeros((0, 6), dtype=np.int32)
@@ -583,11 +616,15 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             win=bufs["win"].data_ptr(), src_coef=bufs["src_coef"].data_ptr(),
         )
         x = torch.zeros(sshape, dtype=torch.float64, device=dev)
+        em_struct, em_bufs = (None, None)
+        if self.enable_em_cascade:
+            em_struct, em_bufs = self._em_setup(zs, K, dv)
         st = _lib.PrhState(
             x=x.data_ptr(), a=bufs["a"].data_ptr(), fphi=bufs["fphi"].data_ptr(),
             step_ctl=bufs["step_ctl"].data_ptr(), rec_x=None, rec_a=None, rec_fx=None,
             rec_fa=None, rec_first=0, rec_count=0,
             wpart=bufs["wpart"].data_ptr() if wide else None,
+            em=ctypes.addressof(em_struct) if em_struct is not None else None,
         )
         self._plan = {
             "op": op, "opT": opT, "sch": sch, "st": st, "x": x, "bufs": bufs,
@@ -595,6 +632,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             "stream": stream, "stride": stride, "dph": cache.dph, "K": K,
             "fold_rows": stride, "graph": None, "two": two, "z_grid": z_grid,
             "src_slots": slots, "J": J, "win": win, "wide": wide,
+            "em": em_struct, "em_bufs": em_bufs,
         }
         if wide:
             self._plan.update(wtasks=wtasks, wheavy=wheavy, nwslots=nwslots)
@@ -604,9 +642,62 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         self.timings["setup_host_s"] = t1 - t0
         return self._plan
 
+    def _em_setup(self, zs, K, dv):
+        """Device buffers of the EM sector: per-window cascade transfers T(z) and
+        Bethe-Heitler shapes at the refresh redshifts (propagation.py:461-618),
+        the per-window pair weights and the photon state."""
+        from ..model.cascade import cascade_kernels
+        from ..model.partial_diff import single_block
+
+        torch = _torch()
+        run = self.prince_run
+        if self.enable_decay:
+            raise NotImplementedError("explicit decay with the EM cascade")
+        eg = run.em_grid
+        if eg.d > _lib.EM_MAX:
+            raise NotImplementedError(f"the EM kernels take at most {_lib.EM_MAX} bins")
+        ck = cascade_kernels(eg.grid, run.cr_grid.grid)
+        field = run.photon_field
+        Tg, Te = ck.transfer(zs, field, self._em_dz)
+        Rbh = ck.bh_shape(zs, field)
+        E_p = run.cr_grid.grid
+        dE = np.gradient(E_p)
+        pair = self.pair_loss_rates_grid
+        bhw = np.stack([
+            (PRINCE_UNITS.c * self._em_dz / ((1.0 + z) * H(z))) * (dE / E_p) * pair.rate_single(z)
+            for z in zs
+        ])
+        cbh = np.zeros(self.spec_man.nspec)
+        for sref in self.spec_man.species_refs:
+            if sref.is_nucleus and abs(sref.charge) >= 1:
+                cbh[sref.princeidx] = float(sref.A) * pair.species_scale[sref.princeidx]
+        dcoef, ddiag = fd_tables(single_block(eg))
+        dev = self.device
+        gam0 = np.zeros((eg.d, K))
+        init = getattr(self, "_em_init", None)
+        if init is not None and init.size:
+            gam0 = np.asarray(init[2 * eg.d : 3 * eg.d], dtype=np.float64).reshape(eg.d, K)
+        b = {
+            "egrid": dv(eg.grid, np.float64), "dcoef": dv(dcoef, np.float64),
+            "ddiag": dv(ddiag, np.float64), "cbh": dv(cbh, np.float64),
+            "bhw": dv(bhw, np.float64), "rbh": dv(Rbh, np.float64),
+            "tg": dv(Tg, np.float64), "te": dv(Te, np.float64),
+            "gam": dv(gam0, np.float64), "gam0": dv(gam0, np.float64),
+            "wbh": torch.zeros((run.cr_grid.d, K), dtype=torch.float64, device=dev),
+        }
+        em = _lib.PrhEm(
+            d_em=eg.d, nwin=len(zs), egrid=b["egrid"].data_ptr(), dcoef=b["dcoef"].data_ptr(),
+            ddiag=b["ddiag"].data_ptr(), cbh=b["cbh"].data_ptr(), bhw=b["bhw"].data_ptr(),
+            rbh=b["rbh"].data_ptr(), tg=b["tg"].data_ptr(), te=b["te"].data_ptr(),
+            gam=b["gam"].data_ptr(), wbh=b["wbh"].data_ptr(),
+        )
+        self.em_iterations = list(getattr(ck, "iterations", []))
+        return em, b
+
     # ---------------------------------------------------------------- device run
     def prepare(self, dz=1e-3, with_adjoint=False):
         """Host-side setup of a solve; every input ends up resident in HBM."""
+        self._em_dz = float(abs(dz))
         z_grid = z_grid_for(self.initial_z, self.final_z, dz)
         self._reset_for_solve()
         self._ensure_D_split()
@@ -632,6 +723,8 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         plan = plan if plan is not None else self._plan
         if k_begin == 0:
             plan["x"].zero_()
+            if plan.get("em_bufs") is not None:
+                plan["em_bufs"]["gam"].zero_()
             self.fold_windows(plan)
         k_end = plan["nsteps"] if k_end is None else k_end
         self._launch_steps(plan, k_begin, k_end)
@@ -792,7 +885,8 @@ class _MultiRHSView:
 
     @property
     def res(self):
-        return UHECRPropagationResult(self.state, self.parent.egrid, self.parent.spec_man)
+        return UHECRPropagationResult(self.state, self.parent.egrid, self.parent.spec_man,
+                                      em=self.parent._em_info())
 
 
 class MultiRHSPropagationSolverETD2(UHECRPropagationSolverETD2):

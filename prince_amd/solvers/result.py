@@ -17,10 +17,14 @@ from ..model.units import EnergyGrid, SpeciesManager, is_nucleus
 
 
 class UHECRPropagationResult:
-    def __init__(self, state, egrid, spec_man):
+    """``em`` maps the PDG ids of EM species (photon, e+-) to (grid, slice of
+    state) when the solve carried the EM cascade; they live on their own grid."""
+
+    def __init__(self, state, egrid, spec_man, em=None):
         self.spec_man = spec_man
         self.egrid = egrid
         self.state = state
+        self.em = dict(em or {})
 
     def to_dict(self):
         return {"egrid": self.egrid, "state": self.state, "known_spec": self.known_species}
@@ -42,7 +46,7 @@ class UHECRPropagationResult:
             )
         if not np.array_equal(self.known_species, other.known_species):
             raise Exception("Cannot add Propagation Results, have different species managers!")
-        return UHECRPropagationResult(self.state + other.state, self.egrid, self.spec_man)
+        return UHECRPropagationResult(self.state + other.state, self.egrid, self.spec_man, em=self.em)
 
     def __mul__(self, number):
         if not np.isscalar(number):
@@ -51,9 +55,12 @@ class UHECRPropagationResult:
                     type(number)
                 )
             )
-        return UHECRPropagationResult(self.state * number, self.egrid, self.spec_man)
+        return UHECRPropagationResult(self.state * number, self.egrid, self.spec_man, em=self.em)
 
     def get_solution(self, pdg_id):
+        if pdg_id in self.em:
+            grid, sl = self.em[pdg_id]
+            return grid, self.state[sl]
         spec = self.spec_man.pdgid2sref[pdg_id]
         return self.egrid, self.state[spec.lidx() : spec.uidx()]
 

@@ -43,6 +43,22 @@ def nuclear_run(ebl_seed=0):
     return _RUNS[key]
 
 
+def proton_em_run():
+    """Protons on the CMB with the co-evolved EM cascade (photons, e+-)."""
+    if "proton_em" not in _RUNS:
+        _RUNS["proton_em"] = PriNCeRun(max_mass=1, photon_field=CMBPhotonSpectrum(), em_cascade=True)
+    return _RUNS["proton_em"]
+
+
+def nuclear_em_run(ebl_seed=0):
+    """The A <= 56 nuclear run (CMB + EBL) with the co-evolved EM cascade."""
+    key = ("nuclear_em", ebl_seed)
+    if key not in _RUNS:
+        pf = CombinedPhotonField([CMBPhotonSpectrum(), SyntheticEBL(seed=ebl_seed)])
+        _RUNS[key] = PriNCeRun(max_mass=56, photon_field=pf, em_cascade=True)
+    return _RUNS[key]
+
+
 def proton_source(run):
     return SimpleSource(run, params={PDG_PROTON: (2.0, 10**11.5, 1.0)})
 

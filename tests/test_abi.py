@@ -61,3 +61,30 @@ def test_backend_dispatch_and_gpu_requirement():
     if not torch.cuda.is_available():
         with pytest.raises(RuntimeError, match="GPU"):
             UHECRPropagationSolverETD2(1.0, 0.0, run)
+
+
+def test_abi_rejects_bad_wide_operators_without_touching_the_gpu():
+    """The wide layout's preconditions are checked before any launch."""
+    import ctypes
+
+    lib = _lib.load()
+    dummy = ctypes.c_int32(0)
+    ptr = ctypes.addressof(dummy)
+    op = _lib.PrhOperator(nspec=2, d=64, K=96, ntasks=0, wide=1, nwtasks=1, wtasks=ptr, kt=1)
+    sch = _lib.PrhSchedule(nsteps=4)
+    st = _lib.PrhState(x=ptr, a=ptr, fphi=ptr, step_ctl=ptr)
+    args = (ctypes.byref(op), ctypes.byref(sch), ctypes.byref(st), 0, 1, None)
+    assert lib.prh_etd2_steps(*args) != 0
+    assert b"K % 64" in lib.prh_last_error()
+    op.K = 128
+    op.nheavy = 1
+    assert lib.prh_etd2_steps(*args) != 0
+    assert b"partial buffer" in lib.prh_last_error()
+    op.nheavy = 0
+    op.nwtasks = 0
+    assert lib.prh_etd2_steps(*args) != 0
+    assert b"empty operator" in lib.prh_last_error()
+    op.nwtasks = 1
+    adj = _lib.PrhAdjoint(lam=ptr, lam_a=ptr, u=ptr, v=ptr)
+    assert lib.prh_adjoint_step(ctypes.byref(op), ctypes.byref(sch), ctypes.byref(adj), 0, None) != 0
+    assert b"narrow layout" in lib.prh_last_error()

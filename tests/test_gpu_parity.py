@@ -103,9 +103,9 @@ def test_kt_and_graph_variants_agree():
         assert np.array_equal(st, base), key
 
 
-def _multi(run, zi, K, srcs, **tuned):
+def _multi(run, zi, K, srcs, tuned=None):
     g = MultiRHSPropagationSolverETD2(zi, 0.0, run, K=K)
-    g._tuned = dict(tuned)
+    g._tuned = dict(tuned or {})
     for k in range(K):
         g[k].add_source_class(srcs[k])
     return g
@@ -116,7 +116,7 @@ def test_wide_layout_matches_oracle_columns():
     run = workloads.nuclear_run()
     K = 64
     srcs = workloads.batch_sources(run, K, seed=3)
-    g = _multi(run, 0.1, K, srcs)
+    g = _multi(run, 0.1, K, srcs, {"layout": "wide"})
     assert g._use_wide()
     r = OracleETD2(0.1, 0.0, run, K=K)
     for k in range(K):
@@ -140,12 +140,12 @@ def test_wide_layout_matches_narrow(part_cost):
     run = workloads.nuclear_run()
     K = 128
     srcs = workloads.batch_sources(run, K, seed=4)
-    narrow = _multi(run, 0.15, K, srcs, layout="narrow")
+    narrow = _multi(run, 0.15, K, srcs, {"layout": "narrow"})
     narrow.solve(dz=1e-3)
     states = []
     for graphs in (True, False):
         run.backend.use_hip_graphs = graphs
-        g = _multi(run, 0.15, K, srcs, layout="wide", **{("wide_part_cost", K): part_cost})
+        g = _multi(run, 0.15, K, srcs, {"layout": "wide", ("wide_part_cost", K): part_cost})
         g.solve(dz=1e-3)
         states.append(g.state.copy())
     run.backend.use_hip_graphs = True

@@ -117,3 +117,14 @@ def test_wide_emulation_matches_offdiagonal_product():
     assert np.all(np.isfinite(rows)) and err.max() < 1e-13
     # the diagonal region of the window holds diag(M) (used by the exponential)
     assert np.array_equal(mv[wl.diag_off : wl.diag_off + lay.n], M.diagonal())
+
+
+def test_nuclear_layout_has_only_wide_shapes():
+    from prince_amd.solvers.layout import wide_shapes_ok
+
+    run = workloads.nuclear_run()
+    lay, _, _ = _layout(run)
+    assert wide_shapes_ok(lay.blocks)
+    bad = lay.blocks.copy()
+    bad[0, 1:3] = (2, 3)
+    assert not wide_shapes_ok(bad)
