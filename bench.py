@@ -53,7 +53,28 @@ def parse():
     p.add_argument("--no-events", action="store_true")
     p.add_argument("--no-adjoint", action="store_true")
     p.add_argument("--adjoint-batch", type=int, default=64)
+    p.add_argument("--z-max", type=float, default=3.0,
+                   help="initial redshift (3.0 is the benchmark; smaller values shorten profiler runs)")
+    p.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "etd2_stage_K512_traffic.json"),
+                   help="rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the stage kernels (JSON)")
     return p.parse_args()
+
+
+def measured_traffic(path, kernel_names):
+    """HBM bytes per stage launch from a committed rocprofv3 counter summary:
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each divided by what the same
+    counter read per byte for an elementwise copy of known size, averaged over
+    the stage kernels."""
+    if not path or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    ks = [v for k, v in rec["kernels"].items() if any(n in k for n in kernel_names)]
+    if not ks:
+        return None
+    per = [1024.0 * (v["FETCH_SIZE_KiB"] / rec["fetch_factor"] + v["WRITE_SIZE_KiB"] / rec["write_factor"])
+           for v in ks]
+    return float(np.mean(per))
 
 
 def timed(fn, torch, reps):
@@ -97,6 +118,14 @@ def stage_bytes(plan, n, K):
     matrix = 8.0 * (lay.fwd.zslots + n) + 16.0 * lay.blocks.shape[0]
     src = plan["bufs"]["src_grid"].numel() * 8.0
     return state + matrix + src
+
+
+def stage_kernel_names(plan):
+    """Names (as rocprofv3 prints them) of the two stage kernels a plan launches."""
+    if plan.get("wide"):
+        return ["etd2_wide<false>", "etd2_wide<true>"]
+    kt, two = plan["kt"], "true" if plan["two"] else "false"
+    return [f"etd2_stage<false, {kt}, {two}>", f"etd2_stage<true, {kt}, {two}>"]
 
 
 def fold_flops(plan):
@@ -200,7 +229,7 @@ def main():
     # scaling the profiler's FETCH_SIZE / WRITE_SIZE counters
     calib_src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
     calib_dst = torch.empty_like(calib_src)
-    calib_dst.copy_(calib_src)
+    torch.mul(calib_src, 1.0, out=calib_dst)  # one elementwise kernel: 256 MiB in, 256 MiB out
     torch.cuda.synchronize()
     del calib_src, calib_dst
     t0 = time.perf_counter()
@@ -209,7 +238,8 @@ def main():
     log(f"model built in {time.perf_counter() - t0:.1f}s (n={n})")
 
     # ---------------- plain path (K = 1)
-    plain = UHECRPropagationSolverETD2(3.0, 0.0, run)
+    zmax = args.z_max
+    plain = UHECRPropagationSolverETD2(zmax, 0.0, run)
     plain.add_source_class(workloads.silicon_source(run))
     pplan = plain.prepare(1e-3)
     plain.execute(pplan)
@@ -221,7 +251,7 @@ def main():
 
     # ---------------- batched path (K = batch)
     K = args.batch
-    multi = MultiRHSPropagationSolverETD2(3.0, 0.0, run, K=K)
+    multi = MultiRHSPropagationSolverETD2(zmax, 0.0, run, K=K)
     srcs = workloads.batch_sources(run, K, seed=0)
     for k in range(K):
         multi[k].add_source_class(srcs[k])
@@ -242,7 +272,7 @@ def main():
     check_cols = [0, K // 3, K - 1]
     col_dev = []
     for k in check_cols:
-        one = UHECRPropagationSolverETD2(3.0, 0.0, run)
+        one = UHECRPropagationSolverETD2(zmax, 0.0, run)
         one.add_source_class(srcs[k])
         one.solve(1e-3)
         ref = one.state
@@ -265,7 +295,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args.traffic, stage_kernel_names(bplan)),
             "kernel": (("etd2_wide + etd2_wide_combine" if bplan.get("wide") else "etd2_stage")
                        + " (stage 1 and 2, batched K=%d)" % K),
             "bytes_per_launch": nbytes,
@@ -302,8 +332,8 @@ def main():
         from oracle.etd2_cpu import OracleETD2
 
         ns = args.cpu_sample_steps
-        zend = 3.0 - ns * 1e-3
-        o = OracleETD2(3.0, zend, run, K=K)
+        zend = zmax - ns * 1e-3
+        o = OracleETD2(zmax, zend, run, K=K)
         for k in range(K):
             o.add_source_class(srcs[k], k)
         with threadpool_limits(limits=1):
@@ -319,7 +349,7 @@ def main():
             "cores": 1,
             "kind": "port",
             "sample": f"CPU FP64 oracle, K={K} batched columns, first {nsam} of {nsteps} steps "
-            f"(z 3 -> {zend:.3f}, {len(o.refresh_zs)} rate refreshes) in {tc:.1f}s, "
+            f"(z {zmax:g} -> {zend:.3f}, {len(o.refresh_zs)} rate refreshes) in {tc:.1f}s, "
             "extrapolated by step count",
         }
         log(f"cpu oracle sample: {tc:.1f}s for {nsam} steps -> {cpu_sps:.3f} solves/s")
@@ -339,7 +369,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"nuclear_batch: A<=56 ({run.spec_man.nspec} species, {n} states), "
-            f"CMB+EBL, K={K} seeded source draws, z 3->0, dz=1e-3",
+            f"CMB+EBL, K={K} seeded source draws, z {zmax:g}->0, dz=1e-3",
             "K": K,
             "nsteps": nsteps,
             "windows": pplan["nwin"],

@@ -3,6 +3,7 @@
 Usage::
 
     python -m prince_amd.rocprof_summary <rocprofv3 output dir> [--pmc COUNTER]
+    python -m prince_amd.rocprof_summary traffic --fetch DIR --write DIR --match NAME ...
 
 Reads every ``*_results.db`` (rocpd SQLite) or ``*kernel_trace.csv`` /
 ``*counter_collection.csv`` under the directory and prints, per kernel: the
@@ -86,7 +87,52 @@ def summarize(path, counters=()):
     return "\n".join(lines)
 
 
+def traffic(fetch_dir, write_dir, match, calib_bytes, calib_match="elementwise"):
+    """Per-dispatch HBM traffic of the kernels whose names contain one of
+    ``match``, from a FETCH_SIZE run and a WRITE_SIZE run (KiB per dispatch).
+
+    The calibration factors are FETCH_SIZE and WRITE_SIZE of the largest
+    kernel whose name contains ``calib_match`` (an elementwise copy that reads
+    and writes ``calib_bytes``) divided by ``calib_bytes``; traffic in bytes is
+    FETCH / fetch_factor + WRITE / write_factor.
+    """
+    fetch = counter_rows(fetch_dir, "FETCH_SIZE")
+    write = counter_rows(write_dir, "WRITE_SIZE")
+    cal = [(max(v), k) for k, v in fetch.items() if calib_match in k]
+    if not cal:
+        raise SystemExit("no calibration kernel in the FETCH_SIZE run")
+    _, cname = max(cal)
+    cf = max(fetch[cname]) * 1024.0
+    cw = max(write.get(cname, [0.0])) * 1024.0
+    out = {"calibration_kernel": _short(cname, 200), "calibration_bytes_each_way": calib_bytes,
+           "calibration_fetch_KiB": cf / 1024.0, "calibration_write_KiB": cw / 1024.0,
+           "fetch_factor": cf / calib_bytes, "write_factor": cw / calib_bytes, "kernels": {}}
+    for name, vals in fetch.items():
+        short = _short(name, 200)
+        if not any(m in short for m in match):
+            continue
+        wv = write.get(name, [])
+        out["kernels"][short] = {
+            "dispatches": len(vals),
+            "FETCH_SIZE_KiB": sum(vals) / len(vals),
+            "WRITE_SIZE_KiB": (sum(wv) / len(wv)) if wv else 0.0,
+        }
+    return out
+
+
 def main():
+    import json
+    import sys
+
+    if len(sys.argv) > 1 and sys.argv[1] == "traffic":
+        ap = argparse.ArgumentParser(prog="rocprof_summary traffic")
+        ap.add_argument("--fetch", required=True)
+        ap.add_argument("--write", required=True)
+        ap.add_argument("--match", action="append", required=True)
+        ap.add_argument("--calib-bytes", type=float, default=256 * 1024 * 1024)
+        a = ap.parse_args(sys.argv[2:])
+        print(json.dumps(traffic(a.fetch, a.write, a.match, a.calib_bytes), indent=1))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("path")
     ap.add_argument("--pmc", action="append", default=[])
