@@ -53,6 +53,8 @@ def parse():
     p.add_argument("--no-events", action="store_true")
     p.add_argument("--no-adjoint", action="store_true")
     p.add_argument("--adjoint-batch", type=int, default=64)
+    p.add_argument("--no-uneven", action="store_true")
+    p.add_argument("--uneven-batch", type=int, default=1024)
     p.add_argument("--z-max", type=float, default=3.0,
                    help="initial redshift (3.0 is the benchmark; smaller values shorten profiler runs)")
     p.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "etd2_stage_K512_traffic.json"),
@@ -205,6 +207,56 @@ def adjoint_leg(K, torch, log):
     }
 
 
+def uneven_leg(run, K, torch, log, seed=0):
+    """Uneven batch (K columns, z_max ~ U[0.1, 5], EBL scale ~ U[0.6, 1.4], the
+    batched source draws): one timed device solve after a 100-step warm-up, and
+    two columns checked against a plain single-column solve of the same field."""
+    from prince_amd import workloads
+    from prince_amd.solvers.propagation import MultiRHSPropagationSolverETD2
+    from prince_amd.solvers.uneven import UnevenBatchSolverETD2
+
+    rng = np.random.default_rng(seed + 1)
+    z_max = rng.uniform(0.1, 5.0, size=K)
+    scale = rng.uniform(0.6, 1.4, size=K)
+    srcs = workloads.batch_sources(run, K, seed=seed)
+    g = UnevenBatchSolverETD2(z_max, run, ebl_scale=scale)
+    for k in range(K):
+        g[k].add_source_class(srcs[k])
+    t0 = time.perf_counter()
+    plan = g.prepare(1e-3)
+    t_setup = time.perf_counter() - t0
+    g.execute(plan, 0, 100)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.execute(plan)
+    torch.cuda.synchronize()
+    t_run = time.perf_counter() - t0
+    nspec, d = run.spec_man.nspec, run.cr_grid.d
+    host = plan["x"].cpu().numpy().transpose(0, 2, 1).reshape(nspec * d, K)
+    state = np.empty_like(host)
+    state[:, plan["order"]] = host
+    devs = []
+    for k in (int(np.argmax(z_max)), int(np.argmin(z_max))):
+        one = MultiRHSPropagationSolverETD2(float(z_max[k]), 0.0, run, K=1)
+        one.set_field_scales(1, [scale[k]])
+        one[0].add_source_class(srcs[k])
+        one.solve(1e-3)
+        ref = one.state[:, 0]
+        m = np.abs(ref) > 1e-30 * np.abs(ref).max()
+        devs.append(float(np.max(np.abs(state[m, k] - ref[m]) / np.abs(ref[m]))))
+    steps = int(plan["col_steps"].sum())
+    log(f"uneven K={K}: {plan['nsteps']} steps (column total {steps}), {plan['nwin']} windows, "
+        f"setup {t_setup:.1f}s, device {t_run*1e3:.0f} ms -> {K/t_run:.1f} solves/s; "
+        f"column check {max(devs):.2e}")
+    return {
+        "K": K, "z_max_range": [0.1, 5.0], "ebl_scale_range": [0.6, 1.4],
+        "global_steps": plan["nsteps"], "column_steps_total": steps, "windows": plan["nwin"],
+        "setup_host_s": round(t_setup, 2), "ms_per_batch": round(t_run * 1e3, 1),
+        "solves_per_s": round(K / t_run, 2),
+        "column_max_rel_dev_vs_two_field_solve": max(devs),
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -324,6 +376,11 @@ def main():
     if not args.no_adjoint:
         adjoint = adjoint_leg(args.adjoint_batch, torch, log)
 
+    # ---------------- uneven batch (per-column z_max and EBL scale)
+    uneven = None
+    if not args.no_uneven:
+        uneven = uneven_leg(run, args.uneven_batch, torch, log)
+
     # ---------------- CPU baseline: oracle on a bounded sample of the batched workload
     cpu = None
     if not args.no_cpu:
@@ -383,6 +440,7 @@ def main():
         "batched_column_max_rel_dev_vs_plain": max(col_dev),
         "adjoint_forward_ratio": adjoint["ratio"] if adjoint else None,
         "adjoint": adjoint,
+        "uneven": uneven,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,

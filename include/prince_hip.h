@@ -122,6 +122,24 @@ typedef struct prh_em {
   double* wbh;          /* [d][K] scratch: Bethe-Heitler weights of the step */
 } prh_em;
 
+/* Uneven batch: every right-hand side k has its own z grid (its own z_max),
+ * refresh windows and coupling values.  Per-step tables are [nsteps][K];
+ * win < 0 marks a finished column (its state is left as it is).  mrhs holds
+ * each column's current window values in the window layout of prh_operator
+ * ([K][mvals_stride], diagonal at diag_off), refreshed by prh_fold_rows. */
+typedef struct prh_uneven {
+  int32_t nsteps;        /* steps of the longest column */
+  int32_t nwin;          /* windows over all columns (rows of rpair) */
+  const double* h;       /* [nsteps][K] z step of column k (negative) */
+  const double* dldz;    /* [nsteps][K] */
+  const double* hadia;   /* [nsteps][K] H(z) * cm2sec (0 when adiabatic losses are off) */
+  const double* finj;    /* [nsteps][K] dl/dz * cm2sec */
+  const int32_t* win;    /* [nsteps][K] window id (row of rpair) or -1 */
+  const double* src_coef;/* [nsteps][J][K] norm * evolution(z_k) per source */
+  const double* rpair;   /* [nwin][d] pair-production rate of each column window */
+  const double* mrhs;    /* [K][mvals_stride] current window values of each column */
+} prh_uneven;
+
 typedef struct prh_state {
   double* x;         /* [nspec][K][d] solution, updated in place */
   double* a;         /* [nspec][K][d] stage-1 vector */
@@ -209,6 +227,31 @@ int32_t prh_adjoint_steps(const prh_operator* opT, const prh_schedule* sch, cons
  * (etd2.py:412-439). */
 int32_t prh_em_step(const prh_operator* op, const prh_schedule* sch, const prh_state* st,
                     int32_t step, void* stream);
+
+/* Photon-field tables for nw (z, s) pairs: out[w][i] = sum_m W[i][m] *
+ * (n_CMB(P[i][m], z_w) + s_w (1 + z_w)^zpow E0[i][m]), the CMB Planck density
+ * planck_pref E^2 / (exp(E / (e_cmb (1+z))) - 1) (zero where the ratio >= 709)
+ * plus a z-separable component tabulated at z = 0.  Gives the per-window photon
+ * vectors (interaction_rates.py:105-116, W = 1, nm = 1) and pair-loss rates
+ * (interaction_rates.py:1277-1312, W = trapezoid weight * phi(xi)/xi^2) of the
+ * uneven batch, whose columns each see CMB + s_k EBL at their own windows. */
+int32_t prh_field_tables(const double* zs, const double* ss, int32_t nw, const double* P,
+                         const double* W, const double* E0, int32_t ni, int32_t nm,
+                         double planck_pref, double e_cmb, double zpow, double* out,
+                         void* stream);
+
+/* Rate fold of listed columns: out[out_rows[c]][r] = sum_j kern[r][j] phot[in_rows[c]][j]
+ * -- the columns whose window changes at a step of the uneven batch
+ * (interaction_rates.py:824-847 per column). */
+int32_t prh_fold_rows(const double* kern, int64_t nrows, int32_t dph, const double* phot,
+                      const int32_t* in_rows, const int32_t* out_rows, int32_t ncol, double* out,
+                      int64_t out_stride, void* stream);
+
+/* Both ETD2 stages of step `step` of the uneven batch (narrow layout, kt 1/2/4).
+ * Each column follows its own reference solve (propagation.py:813-890 with
+ * etd2.py:412-439); finished columns are skipped. */
+int32_t prh_uneven_step(const prh_operator* op, const prh_uneven* un, const prh_state* st,
+                        int32_t step, void* stream);
 
 /* u = h phi2 lam for step `step`, before the first prh_adjoint_step of a sweep. */
 int32_t prh_adjoint_seed(const prh_operator* opT, const prh_schedule* sch, const prh_adjoint* adj,

@@ -162,6 +162,7 @@ class OracleETD2:
         enable_partial_diff_jacobian=True,
         z_offset=0.0,
         K=None,
+        pfield=None,
     ):
         self.initial_z = initial_z + z_offset
         self.final_z = final_z + z_offset
@@ -173,6 +174,9 @@ class OracleETD2:
         self.enable_injection_jacobian = enable_injection_jacobian
         self.enable_partial_diff_jacobian = enable_partial_diff_jacobian
         self.K = K
+        # photon field of the solve (None: the run's own); lets one oracle
+        # solve stand for a column that sees a rescaled EBL
+        self.pfield = pfield
         ncol = 1 if K is None else int(K)
         self.sources = [[] for _ in range(ncol)]
         self.dim_states = prince_run.dim_states
@@ -221,7 +225,7 @@ class OracleETD2:
     def _refresh(self, z):
         ir = self.run.int_rates
         B = ir.batch_matrix
-        ph = ir.photon_vector(z)
+        ph = ir.photon_vector(z, pfield=self.pfield)
         if self._M_off is None:
             M = ir.coupling_mat.copy()
             M.data = B @ ph
@@ -244,7 +248,7 @@ class OracleETD2:
                     dmg[self._diag_present] = self._diag_kernel.dot(ph)
                 self._M_diag = dmg
         if self.enable_pairprod_losses:
-            self._kappa_pair = self.run.pair_loss_rates_grid.loss_vector(z)
+            self._kappa_pair = self.run.pair_loss_rates_grid.loss_vector(z, pfield=self.pfield)
         else:
             self._kappa_pair = None
         if self.em:

@@ -26,6 +26,9 @@ EXPORTED = (
     "prh_adjoint_seed",
     "prh_adjoint_steps",
     "prh_em_step",
+    "prh_field_tables",
+    "prh_fold_rows",
+    "prh_uneven_step",
 )
 
 WG_WAVES = 16
@@ -122,6 +125,21 @@ class PrhEm(ctypes.Structure):
 EM_MAX = 144
 
 
+class PrhUneven(ctypes.Structure):
+    _fields_ = [
+        ("nsteps", _i32),
+        ("nwin", _i32),
+        ("h", _p),
+        ("dldz", _p),
+        ("hadia", _p),
+        ("finj", _p),
+        ("win", _p),
+        ("src_coef", _p),
+        ("rpair", _p),
+        ("mrhs", _p),
+    ]
+
+
 class PrhAdjoint(ctypes.Structure):
     _fields_ = [
         ("lam", _p),
@@ -185,6 +203,19 @@ def load():
         ctypes.POINTER(PrhSchedule),
         ctypes.POINTER(PrhState),
         _i32,
+        _i32,
+        _p,
+    ]
+    _d = ctypes.c_double
+    lib.prh_field_tables.restype = _i32
+    lib.prh_field_tables.argtypes = [_p, _p, _i32, _p, _p, _p, _i32, _i32, _d, _d, _d, _p, _p]
+    lib.prh_fold_rows.restype = _i32
+    lib.prh_fold_rows.argtypes = [_p, _i64, _i32, _p, _p, _p, _i32, _p, _i64, _p]
+    lib.prh_uneven_step.restype = _i32
+    lib.prh_uneven_step.argtypes = [
+        ctypes.POINTER(PrhOperator),
+        ctypes.POINTER(PrhUneven),
+        ctypes.POINTER(PrhState),
         _i32,
         _p,
     ]

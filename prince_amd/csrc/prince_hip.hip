@@ -1194,6 +1194,234 @@ int32_t validate_em(const prh_operator* op, const prh_state* st, const char* who
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Uneven batch: every right-hand side has its own z grid (its own z_max), hence
+// its own refresh windows, coupling values and per-step scalars; columns
+// retire after their last step.  Narrow layout [species][rhs][bin], lanes = bins.
+// ---------------------------------------------------------------------------
+
+// Photon-field tables for a list of (z, s): out[w][i] = sum_m W[i][m] *
+// (n_CMB(P[i][m], z_w) + s_w (1 + z_w)^zpow E0[i][m]) -- the CMB Planck
+// spectrum evaluated here (photonfields.py:114-126) plus a z-separable component
+// tabulated at z = 0.  One thread per (w, i).
+__global__ void field_tables_kernel(const double* __restrict__ zs, const double* __restrict__ ss,
+                                    int nw, const double* __restrict__ P,
+                                    const double* __restrict__ W, const double* __restrict__ E0,
+                                    int ni, int nm, double planck_pref, double e_cmb, double zpow,
+                                    double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nw * ni) return;
+  const int w = (int)(t / ni);
+  const int i = (int)(t % ni);
+  const double z = zs[w];
+  const double s = ss[w];
+  const double zp1 = 1.0 + z;
+  const double pref = planck_pref * (zp1 * zp1);
+  const double sz = s * pow(zp1, zpow);
+  double acc = 0.0;
+  for (int m = 0; m < nm; ++m) {
+    const int64_t q = (int64_t)i * nm + m;
+    const double Ered = P[q] / zp1;
+    const double ratio = Ered / e_cmb;
+    double n = 0.0;
+    if (ratio < 709.0) n = pref * (Ered * Ered) / (exp(ratio) - 1.0);
+    n = n + sz * E0[q];
+    acc += W[q] * n;
+  }
+  out[t] = acc;
+}
+
+// Rate fold of listed columns: out[orow[c]][r] = sum_j kern[r][j] phot[irow[c]][j]
+// (the fold_gemm_mfma tiling with gathered input and scattered output rows).
+__global__ void __launch_bounds__(THREADS) fold_rows_mfma(const double* __restrict__ kern,
+                                                          int64_t nrows, int dph,
+                                                          const double* __restrict__ phot,
+                                                          const int32_t* __restrict__ irow,
+                                                          const int32_t* __restrict__ orow,
+                                                          int ncol, double* __restrict__ out,
+                                                          int64_t out_stride) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t wave = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x / WAVE);
+  const int64_t r0 = wave * 16;
+  if (r0 >= nrows) return;
+  const int col = lane & 15;
+  const int kq = lane >> 4;
+  const int64_t r = r0 + col;
+  const int nk = (dph + 3) / 4;
+  double bfrag[FOLD_MAX_KSTEPS];
+#pragma unroll
+  for (int t = 0; t < FOLD_MAX_KSTEPS; ++t) {
+    const int j = 4 * t + kq;
+    bfrag[t] = (t < nk && j < dph && r < nrows) ? kern[r * dph + j] : 0.0;
+  }
+  for (int c0 = 0; c0 < ncol; c0 += 16) {
+    double4_t acc = {0.0, 0.0, 0.0, 0.0};
+    const int ca = c0 + col;
+    const int64_t prow = ca < ncol ? (int64_t)irow[ca] * dph : 0;
+#pragma unroll
+    for (int t = 0; t < FOLD_MAX_KSTEPS; ++t) {
+      if (t < nk) {
+        const int j = 4 * t + kq;
+        const double a = (ca < ncol && j < dph) ? phot[prow + j] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bfrag[t], acc, 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int c = c0 + kq + 4 * v;
+      if (c < ncol && r < nrows) out[(int64_t)orow[c] * out_stride + r] = acc[v];
+    }
+  }
+}
+
+template <bool STAGE2, int KT>
+__global__ void __launch_bounds__(STHREADS) etd2_uneven(prh_operator op, prh_uneven un,
+                                                        prh_state st, int step) {
+  __shared__ double red[SWG][KT][WAVE];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = threadIdx.x / WAVE;
+  const int K = op.K;
+  const int d = op.d;
+  int gx, gy;
+  wg_coords(op.ntasks, K, KT, gx, gy);
+  const int k0 = gy * KT;
+  // per-column window of this step (-1: the column has finished); a workgroup
+  // whose columns have all finished leaves before any barrier
+  int wk[KT];
+  bool any = false;
+#pragma unroll
+  for (int kk = 0; kk < KT; ++kk) {
+    const int k = k0 + kk;
+    wk[kk] = (k < K) ? un.win[(int64_t)step * K + k] : -1;
+    any = any || wk[kk] >= 0;
+  }
+  if (!any) return;
+  const int* tk = op.tasks + 6 * ((int64_t)gx * SWG + wv);
+  const int da = tk[0];
+  const bool active = lane < d;
+  const double* __restrict__ xin = STAGE2 ? st.a : st.x;
+  const int64_t stride = op.mvals_stride;
+  double acc[KT];
+#pragma unroll
+  for (int kk = 0; kk < KT; ++kk) acc[kk] = 0.0;
+  if (da >= 0) {
+    for (int b = tk[1]; b < tk[2]; ++b) {
+      const int* blk = op.blocks + 4 * b;
+      const int mo = blk[0];
+      const int olo = blk[1];
+      const int ns = blk[2];
+      const int64_t voff = blk[3];
+      double y[KT];
+#pragma unroll
+      for (int kk = 0; kk < KT; ++kk) {
+        const int k = k0 + kk;
+        y[kk] = (active && wk[kk] >= 0) ? xin[((int64_t)mo * K + k) * d + lane] : 0.0;
+      }
+      for (int s2 = 0; s2 < ns; ++s2) {
+        const int src = lane + olo + s2;
+#pragma unroll
+        for (int kk = 0; kk < KT; ++kk) {
+          const int k = k0 + kk;
+          const double m = (active && wk[kk] >= 0)
+                               ? un.mrhs[(int64_t)k * stride + voff + (int64_t)s2 * d + lane]
+                               : 0.0;
+          acc[kk] += m * shfl_or_zero(y[kk], src, d);
+        }
+      }
+    }
+  }
+  reduce_parts<KT>(red, wv, lane, tk[3], tk[4], acc);
+  if (da < 0 || wv != tk[3]) return;
+
+  const bool pdiff = (op.flags & PRH_F_PARTIAL_DIFF) != 0;
+  const bool inj = (op.flags & PRH_F_INJECTION) != 0 && op.src_slot[da] >= 0;
+  double dcf[7];
+#pragma unroll
+  for (int o = 0; o < 7; ++o) dcf[o] = (active && pdiff) ? op.dcoef[lane * 7 + o] : 0.0;
+  const int slot = inj ? op.src_slot[da] : 0;
+  const int J = op.nsrc;
+  const double E = active ? op.egrid[lane] : 0.0;
+  const double ddg = active ? op.ddiag[lane] : 0.0;
+  const double psc = op.pscale[da];
+#pragma unroll
+  for (int kk = 0; kk < KT; ++kk) {
+    const int k = k0 + kk;
+    if (wk[kk] < 0) continue;  // uniform over the wave
+    const int64_t sk = (int64_t)step * K + k;
+    const double dl = un.dldz[sk];
+    const double hh = un.h[sk];
+    double kap = un.hadia[sk] * E;
+    kap = kap + psc * (active ? un.rpair[(int64_t)wk[kk] * d + lane] : 0.0);
+    const double mdc = active ? un.mrhs[(int64_t)k * stride + op.diag_off + (int64_t)da * d + lane] : 0.0;
+    double dval = mdc * dl;
+    if (pdiff) {
+      double kxd = kap * ddg;
+      kxd = kxd * dl;
+      dval = dval + kxd;
+    }
+    const double hd = dval * hh;
+    const double e = exp(hd);
+    const int64_t own = ((int64_t)da * K + k) * d + lane;
+    const double xown = active ? xin[own] : 0.0;
+    double out = acc[kk];
+    if (pdiff) {
+      const double kx = kap * xown;
+      double dx = 0.0;
+#pragma unroll
+      for (int o = -3; o <= 3; ++o) {
+        if (o == 0) continue;
+        const double kxo = shfl_or_zero(kx, lane + o, d);
+        if (dcf[o + 3] != 0.0) dx += dcf[o + 3] * kxo;
+      }
+      out = out + dx;
+    }
+    out = out * dl;
+    if (inj && active) {
+      double bsum = 0.0;
+      for (int j = 0; j < J; ++j) {
+        const double c = un.src_coef[((int64_t)step * J + j) * K + k];
+        const double g = op.src_grid[(((int64_t)slot * J + j) * K + k) * d + lane];
+        bsum = (j == 0) ? c * g : bsum + c * g;
+      }
+      out = out + un.finj[sk] * bsum;
+    }
+    if (!active) continue;
+    if (!STAGE2) {
+      const double phi = phi1_of(hd, e);
+      double a = e * xown;
+      double sc = phi * out;
+      sc = sc * hh;
+      a = a + sc;
+      st.a[own] = a;
+      st.fphi[own] = out;
+    } else {
+      const double phi = phi2_of(hd, e);
+      double sc = out - st.fphi[own];
+      sc = sc * hh;
+      sc = sc * phi;
+      st.x[own] = xown + sc;
+    }
+  }
+}
+
+template <bool S2>
+void launch_uneven(const prh_operator& op, const prh_uneven& un, const prh_state& st, int step,
+                   hipStream_t s) {
+  const int64_t nwg = (int64_t)(op.ntasks / SWG) * ((op.K + op.kt - 1) / op.kt);
+  const dim3 grid((unsigned)nwg);
+  switch (op.kt) {
+    case 1:
+      hipLaunchKernelGGL((etd2_uneven<S2, 1>), grid, dim3(STHREADS), 0, s, op, un, st, step);
+      break;
+    case 2:
+      hipLaunchKernelGGL((etd2_uneven<S2, 2>), grid, dim3(STHREADS), 0, s, op, un, st, step);
+      break;
+    default:
+      hipLaunchKernelGGL((etd2_uneven<S2, 4>), grid, dim3(STHREADS), 0, s, op, un, st, step);
+      break;
+  }
+}
+
 int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_state* st,
                  const char* who) {
   if (!op || !sch || !st) return set_error(std::string(who) + ": null argument");
@@ -1479,6 +1707,57 @@ int32_t prh_graph_destroy(prh_graph* g) {
   if (g->graph) hipGraphDestroy(g->graph);
   delete g;
   return 0;
+}
+
+
+int32_t prh_field_tables(const double* zs, const double* ss, int32_t nw, const double* P,
+                         const double* W, const double* E0, int32_t ni, int32_t nm,
+                         double planck_pref, double e_cmb, double zpow, double* out,
+                         void* stream) {
+  if (nw < 0 || ni < 0 || nm < 0) return set_error("prh_field_tables: negative size");
+  if ((int64_t)nw * ni == 0) return 0;
+  if (!zs || !ss || !P || !W || !E0 || !out) return set_error("prh_field_tables: null pointer");
+  const int64_t n = (int64_t)nw * ni;
+  hipLaunchKernelGGL(field_tables_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, zs, ss, (int)nw, P, W, E0, (int)ni, (int)nm,
+                     planck_pref, e_cmb, zpow, out);
+  return check_hip(hipGetLastError(), "prh_field_tables launch");
+}
+
+int32_t prh_fold_rows(const double* kern, int64_t nrows, int32_t dph, const double* phot,
+                      const int32_t* in_rows, const int32_t* out_rows, int32_t ncol, double* out,
+                      int64_t out_stride, void* stream) {
+  if (nrows < 0 || ncol < 0) return set_error("prh_fold_rows: negative size");
+  if (dph <= 0 || dph > 4 * FOLD_MAX_KSTEPS) return set_error("prh_fold_rows: dph must be in [1, 128]");
+  if (out_stride < nrows) return set_error("prh_fold_rows: out_stride < nrows");
+  if (nrows == 0 || ncol == 0) return 0;
+  if (!kern || !phot || !in_rows || !out_rows || !out) return set_error("prh_fold_rows: null pointer");
+  const int64_t waves = (nrows + 15) / 16;
+  const int64_t blocks = (waves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+  hipLaunchKernelGGL(fold_rows_mfma, dim3((unsigned)blocks), dim3(THREADS), 0, (hipStream_t)stream,
+                     kern, nrows, (int)dph, phot, in_rows, out_rows, (int)ncol, out, out_stride);
+  return check_hip(hipGetLastError(), "prh_fold_rows launch");
+}
+
+int32_t prh_uneven_step(const prh_operator* op, const prh_uneven* un, const prh_state* st,
+                        int32_t step, void* stream) {
+  if (!op || !un || !st) return set_error("prh_uneven_step: null argument");
+  if (op->d <= 0 || op->d > WAVE) return set_error("prh_uneven_step: d must be in [1, 64]");
+  if (op->K <= 0) return set_error("prh_uneven_step: K must be >= 1");
+  if (op->ntasks <= 0 || op->ntasks % SWG != 0)
+    return set_error("prh_uneven_step: ntasks must be a positive multiple of 16");
+  if (!(op->kt == 1 || op->kt == 2 || op->kt == 4))
+    return set_error("prh_uneven_step: kt must be 1, 2 or 4");
+  if (op->wide || op->mvals_e) return set_error("prh_uneven_step: narrow one-field operators only");
+  if (step < 0 || step >= un->nsteps) return set_error("prh_uneven_step: step outside the schedule");
+  if (!un->h || !un->dldz || !un->hadia || !un->finj || !un->win || !un->rpair || !un->mrhs)
+    return set_error("prh_uneven_step: null schedule buffer");
+  if ((op->flags & PRH_F_INJECTION) && !un->src_coef)
+    return set_error("prh_uneven_step: injection needs src_coef");
+  hipStream_t s = (hipStream_t)stream;
+  launch_uneven<false>(*op, *un, *st, step, s);
+  launch_uneven<true>(*op, *un, *st, step, s);
+  return check_hip(hipGetLastError(), "prh_uneven_step launch");
 }
 
 }  // extern "C"

@@ -1,0 +1,307 @@
+"""Uneven batch: K independent ETD2 solves with their own initial redshifts and
+photon-field scales, advanced together on one GPU.
+
+Column k is exactly the reference solve ``UHECRPropagationSolverETD2(z_max[k],
+final_z, run)`` (``propagation.py:813-890``) on the field ``CMB + s_k * EBL``:
+its own z grid (``propagation.py:826-832``), its own refresh windows
+(``propagation.py:1383-1387``) and therefore its own coupling values.  The
+device keeps one window of coupling values per column ([K][window layout]);
+at every step the columns that cross a window boundary are re-folded
+(``prh_fold_rows``: one MFMA GEMM over just those columns), then both stages run
+for every column that has steps left (``prh_uneven_step``); finished columns
+keep their final state.  Columns are ordered by decreasing step count, so the
+workgroups of finished columns leave at once.
+
+Per-window photon vectors and pair-loss rates of every column come from
+``prh_field_tables`` (the CMB Planck spectrum evaluated on the device plus the
+z-separable EBL tabulated at z = 0).  The EM cascade is not part of this path.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+
+from .. import _lib
+from ..model import config as mconfig
+from ..model.cosmology import H
+from ..model.photonfields import PLANCK_PREF, CMBPhotonSpectrum, CombinedPhotonField, SyntheticEBL
+from ..model.units import PRINCE_UNITS
+from .layout import build_tasks, fd_tables
+from .propagation import (
+    MultiRHSPropagationSolverETD2,
+    _MultiRHSETD2SolverHIP,
+    _torch,
+    refresh_schedule,
+    z_grid_for,
+)
+
+
+def _field_parts(field):
+    """(CMB component, EBL component) of a CMB + SyntheticEBL field."""
+    if not isinstance(field, CombinedPhotonField):
+        raise NotImplementedError("the uneven batch needs a CombinedPhotonField(CMB, SyntheticEBL)")
+    cmb = [m for m in field.model_list if isinstance(m, CMBPhotonSpectrum)]
+    ebl = [m for m in field.model_list if isinstance(m, SyntheticEBL)]
+    if len(cmb) != 1 or len(ebl) != 1 or len(field.model_list) != 2:
+        raise NotImplementedError("the uneven batch needs exactly one CMB and one SyntheticEBL")
+    return cmb[0], ebl[0]
+
+
+def column_field(field, scale):
+    """The field a column with EBL scale ``scale`` sees (host form, for checks)."""
+    _, ebl = _field_parts(field)
+    return CombinedPhotonField(
+        [CMBPhotonSpectrum(), SyntheticEBL(amplitudes=ebl.amplitudes, scale=ebl.scale * scale)]
+    )
+
+
+class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
+    """K columns with initial redshifts ``z_max[k]`` and EBL scales ``ebl_scale[k]``.
+
+    Sources per column via ``solver[k].add_source_class(src)``; ``solve(dz)``
+    leaves the final states in ``state`` ([dim_states, K], column order as
+    given).
+    """
+
+    #: right-hand sides per wave of the uneven stage kernel
+    kt_uneven = 4
+
+    def __new__(cls, *args, **kwargs):
+        return object.__new__(cls)
+
+    def __init__(self, z_max, prince_run, final_z=0.0, ebl_scale=None, **kwargs):
+        z_max = np.asarray(z_max, dtype=np.float64).reshape(-1)
+        K = z_max.size
+        if np.any(z_max <= final_z):
+            raise ValueError("every z_max must lie above final_z")
+        MultiRHSPropagationSolverETD2.__init__(self, float(z_max.max()), final_z, prince_run, K=K,
+                                               **kwargs)
+        if self.enable_em_cascade:
+            raise NotImplementedError("the uneven batch does not carry the EM cascade")
+        self.z_max = z_max
+        self.ebl_scale = (np.ones(K) if ebl_scale is None
+                          else np.asarray(ebl_scale, dtype=np.float64).reshape(-1))
+        if self.ebl_scale.size != K:
+            raise ValueError("need one EBL scale per column")
+        _field_parts(prince_run.photon_field)
+
+    # ---------------------------------------------------------------- schedule
+    def _column_schedules(self, dz):
+        """Per-column z grids and windows, ordered by decreasing step count."""
+        cols = []
+        for k in range(self._K):
+            zg = z_grid_for(float(self.z_max[k]), self.final_z, dz)
+            win, zs = refresh_schedule(zg, self.recomp_z_threshold)
+            cols.append((zg, np.asarray(win), np.asarray(zs)))
+        order = np.argsort([-(len(c[0]) - 1) for c in cols], kind="stable")
+        return cols, order
+
+    def _device_tables(self, zw, sw, dev):
+        """Photon vectors [nwin][dph] and pair rates [nwin][d] of all column windows."""
+        torch = _torch()
+        run = self.prince_run
+        cmb, ebl = _field_parts(run.photon_field)
+        ir = self.had_int_rates
+        pair = self.pair_loss_rates_grid
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        zt = torch.from_numpy(np.ascontiguousarray(zw)).to(dev)
+        st = torch.from_numpy(np.ascontiguousarray(sw * ebl.scale)).to(dev)
+        nw = zw.size
+
+        def table(P, W, E0, ni, nm):
+            Pt = torch.from_numpy(np.ascontiguousarray(P, dtype=np.float64)).to(dev)
+            Wt = torch.from_numpy(np.ascontiguousarray(W, dtype=np.float64)).to(dev)
+            Et = torch.from_numpy(np.ascontiguousarray(E0, dtype=np.float64)).to(dev)
+            out = torch.empty((nw, ni), dtype=torch.float64, device=dev)
+            _lib.check(self._lib.prh_field_tables(
+                zt.data_ptr(), st.data_ptr(), nw, Pt.data_ptr(), Wt.data_ptr(), Et.data_ptr(),
+                ni, nm, PLANCK_PREF, mconfig.E_CMB, SyntheticEBL.Z_POWER, out.data_ptr(), stream),
+                "field tables")
+            return out
+
+        Eph = ir.e_photon.grid
+        phot = table(Eph, np.ones_like(Eph), ebl.density_z0(Eph), Eph.size, 1)
+        pg = pair.photon_grid  # [d][nxi]
+        tw = np.empty_like(pair.xi)
+        tw[1:-1] = 0.5 * (pair.xi[2:] - pair.xi[:-2])
+        tw[0] = 0.5 * (pair.xi[1] - pair.xi[0])
+        tw[-1] = 0.5 * (pair.xi[-1] - pair.xi[-2])
+        W = np.broadcast_to(tw * pair.phi_xi2, pg.shape)
+        rpair = table(pg, W, ebl.density_z0(pg), pg.shape[0], pg.shape[1])
+        return phot, rpair
+
+    def prepare(self, dz=1e-3, with_adjoint=False):
+        if with_adjoint:
+            raise NotImplementedError("the uneven batch has no adjoint")
+        torch = _torch()
+        t0 = time.perf_counter()
+        self._reset_for_solve()
+        self._ensure_D_split()
+        dev = self.device
+        K = self._K
+        run = self.prince_run
+        nspec, d = self.spec_man.nspec, run.cr_grid.d
+        cols, order = self._column_schedules(dz)
+        nsteps = np.array([len(cols[k][0]) - 1 for k in order])
+        T = int(nsteps.max())
+        # per-step tables in device column order
+        h = np.zeros((T, K))
+        dldz = np.zeros((T, K))
+        hadia = np.zeros((T, K))
+        finj = np.zeros((T, K))
+        gwin = np.full((T, K), -1, dtype=np.int32)
+        col_sources = self._column_sources()
+        col_sources = [col_sources[k] for k in order]
+        J = max(1, max(len(c) for c in col_sources))
+        coef = np.zeros((T, J, K))
+        zw, sw = [], []
+        ref_in, ref_out, ref_off = [], [], [0]
+        off = 0
+        firsts = []
+        for c, k in enumerate(order):
+            zg, win, zs = cols[k]
+            n = len(zg) - 1
+            z0 = zg[:-1]
+            h[:n, c] = np.diff(zg)
+            # scalar evaluation per step, as the plain path (numpy's vector pow and
+            # sqrt round differently from the scalar ones)
+            dl = [self.dldz(z) for z in z0]
+            dldz[:n, c] = dl
+            if self.enable_adiabatic_losses and self.enable_partial_diff_jacobian:
+                hadia[:n, c] = [H(z) * PRINCE_UNITS.cm2sec for z in z0]
+            finj[:n, c] = [v * 1.0 * PRINCE_UNITS.cm2sec for v in dl]
+            gwin[:n, c] = off + win
+            for j, src in enumerate(col_sources[c]):
+                coef[:n, j, c] = [src.norm * src.evolution(z) for z in z0]
+            zw.extend(zs)
+            sw.extend([self.ebl_scale[k]] * len(zs))
+            firsts.append(off)
+            off += len(zs)
+        # refresh lists: columns whose window changes at step t
+        for t in range(T):
+            prev = gwin[t - 1] if t > 0 else np.full(K, -1)
+            cur = gwin[t]
+            chg = np.flatnonzero((cur >= 0) & (cur != prev))
+            ref_in.extend(cur[chg].tolist())
+            ref_out.extend(chg.tolist())
+            ref_off.append(len(ref_in))
+        zw = np.asarray(zw)
+        sw = np.asarray(sw)
+        phot, rpair = self._device_tables(zw, sw, dev)
+        if not self.enable_pairprod_losses:
+            rpair.zero_()
+        # operator pattern from the first window of the first column; every
+        # column's first window must have the same non-zero photon bins
+        ph_host = phot.cpu().numpy()
+        nz0 = ph_host[firsts[0]] != 0
+        if any(not np.array_equal(ph_host[f] != 0, nz0) for f in firsts):
+            raise NotImplementedError("columns whose first windows differ in their non-zero photon bins")
+        cache = self._device_cache()
+        lay = self._layout_for(cache, ph_host[firsts[0]])
+        wl = lay["layout"]
+        stride = wl.stride
+        tasks = build_tasks(wl.fwd, self.part_cost_for(K), _lib.WG_WAVES)
+        kt = min(self.kt_uneven, 4)
+        while kt > K:
+            kt //= 2
+        src_slot, src_grid, slots = self._source_tables(col_sources, J)
+        dcoef, ddiag = self._D_tables
+        pscale = np.asarray(self.pair_loss_rates_grid.species_scale, dtype=np.float64)
+        has_src = any(len(c) for c in col_sources) and self.enable_injection_jacobian
+
+        def dv(a, dtype=None):
+            a = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype=dtype))
+            return torch.from_numpy(a).to(dev)
+
+        flags = 0
+        if self.enable_partial_diff_jacobian:
+            flags |= _lib.PRH_F_PARTIAL_DIFF
+        if has_src:
+            flags |= _lib.PRH_F_INJECTION
+        bufs = {
+            "tasks": dv(tasks, np.int32),
+            "blocks": dv(wl.blocks if wl.blocks.size else np.zeros((1, 4)), np.int32),
+            "B_sel": lay["B_sel"], "phot": phot, "rpair": rpair,
+            "mrhs": torch.zeros((K, stride), dtype=torch.float64, device=dev),
+            "egrid": dv(run.cr_grid.grid, np.float64), "dcoef": dv(dcoef, np.float64),
+            "ddiag": dv(ddiag, np.float64), "pscale": dv(pscale, np.float64),
+            "src_slot": dv(src_slot, np.int32), "src_grid": dv(src_grid, np.float64),
+            "h": dv(h, np.float64), "dldz": dv(dldz, np.float64), "hadia": dv(hadia, np.float64),
+            "finj": dv(finj, np.float64), "win": dv(gwin, np.int32), "src_coef": dv(coef, np.float64),
+            "ref_in": dv(np.asarray(ref_in + [0]), np.int32),
+            "ref_out": dv(np.asarray(ref_out + [0]), np.int32),
+            "a": torch.zeros((nspec, K, d), dtype=torch.float64, device=dev),
+            "fphi": torch.zeros((nspec, K, d), dtype=torch.float64, device=dev),
+            "step_ctl": torch.zeros(2, dtype=torch.int32, device=dev),
+        }
+        x = torch.zeros((nspec, K, d), dtype=torch.float64, device=dev)
+        op = _lib.PrhOperator(
+            nspec=nspec, d=d, K=K, ntasks=tasks.shape[0], tasks=bufs["tasks"].data_ptr(),
+            blocks=bufs["blocks"].data_ptr(), mvals_stride=stride, diag_off=wl.diag_off,
+            mvals=bufs["mrhs"].data_ptr(), egrid=bufs["egrid"].data_ptr(),
+            dcoef=bufs["dcoef"].data_ptr(), ddiag=bufs["ddiag"].data_ptr(),
+            pscale=bufs["pscale"].data_ptr(), rpair=bufs["rpair"].data_ptr(),
+            src_slot=bufs["src_slot"].data_ptr(), nsrc=J, flags=flags,
+            src_grid=bufs["src_grid"].data_ptr(), kt=kt, pad_=0,
+        )
+        un = _lib.PrhUneven(
+            nsteps=T, nwin=int(zw.size), h=bufs["h"].data_ptr(), dldz=bufs["dldz"].data_ptr(),
+            hadia=bufs["hadia"].data_ptr(), finj=bufs["finj"].data_ptr(),
+            win=bufs["win"].data_ptr(), src_coef=bufs["src_coef"].data_ptr(),
+            rpair=bufs["rpair"].data_ptr(), mrhs=bufs["mrhs"].data_ptr(),
+        )
+        st = _lib.PrhState(x=x.data_ptr(), a=bufs["a"].data_ptr(), fphi=bufs["fphi"].data_ptr(),
+                           step_ctl=bufs["step_ctl"].data_ptr())
+        self._state_wide = False
+        self._plan = {
+            "op": op, "un": un, "st": st, "x": x, "bufs": bufs, "nsteps": T, "order": order,
+            "ref_off": np.asarray(ref_off), "stride": stride, "dph": cache.dph, "K": K,
+            "stream": torch.cuda.current_stream(dev).cuda_stream, "layout": wl, "kt": kt,
+            "nwin": int(zw.size), "col_steps": nsteps, "graph": None, "em_bufs": None,
+            "two": False, "wide": False,
+        }
+        self.timings["setup_host_s"] = time.perf_counter() - t0
+        return self._plan
+
+    def execute(self, plan=None, k_begin=0, k_end=None):
+        """Device part of the batch: zero state, then per step the re-folds and both stages."""
+        plan = plan if plan is not None else self._plan
+        b = plan["bufs"]
+        lib = self._lib
+        stream = plan["stream"]
+        k_end = plan["nsteps"] if k_end is None else k_end
+        if k_begin == 0:
+            plan["x"].zero_()
+        ro = plan["ref_off"]
+        isz = ctypes.sizeof(ctypes.c_int32)
+        for t in range(k_begin, k_end):
+            n = int(ro[t + 1] - ro[t])
+            if n and self.enable_photohad_losses:
+                _lib.check(lib.prh_fold_rows(
+                    b["B_sel"].data_ptr(), plan["stride"], plan["dph"], b["phot"].data_ptr(),
+                    b["ref_in"].data_ptr() + int(ro[t]) * isz, b["ref_out"].data_ptr() + int(ro[t]) * isz,
+                    n, b["mrhs"].data_ptr(), plan["stride"], stream), "column re-fold")
+            _lib.check(lib.prh_uneven_step(ctypes.byref(plan["op"]), ctypes.byref(plan["un"]),
+                                           ctypes.byref(plan["st"]), t, stream), "uneven step")
+        return plan["x"]
+
+    def solve(self, dz=1e-3, verbose=False, summary=False, progressbar=False):
+        torch = _torch()
+        start = time.time()
+        plan = self.prepare(dz)
+        self.execute(plan)
+        torch.cuda.synchronize(self.device)
+        nspec, d, K = self.spec_man.nspec, self.prince_run.cr_grid.d, self._K
+        host = plan["x"].cpu().numpy().transpose(0, 2, 1).reshape(nspec * d, K)
+        state = np.empty_like(host)
+        state[:, plan["order"]] = host
+        self.state = state
+        self.result = None
+        self.last_solve_seconds = time.time() - start
+        if summary or verbose:
+            print(f"uneven batch: {K} columns, {plan['nsteps']} steps, {plan['nwin']} windows, "
+                  f"{self.last_solve_seconds:.3f} s")
+        return state
