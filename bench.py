@@ -241,7 +241,7 @@ def uneven_leg(run, K, torch, log, seed=0):
         one.set_field_scales(1, [scale[k]])
         one[0].add_source_class(srcs[k])
         one.solve(1e-3)
-        ref = one.state[:, 0]
+        ref = np.asarray(one.state).reshape(-1)
         m = np.abs(ref) > 1e-30 * np.abs(ref).max()
         devs.append(float(np.max(np.abs(state[m, k] - ref[m]) / np.abs(ref[m]))))
     steps = int(plan["col_steps"].sum())
