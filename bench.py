@@ -127,7 +127,8 @@ def stage_kernel_names(plan):
     if plan.get("wide"):
         return ["etd2_wide<false>", "etd2_wide<true>"]
     kt, two = plan["kt"], "true" if plan["two"] else "false"
-    return [f"etd2_stage<false, {kt}, {two}>", f"etd2_stage<true, {kt}, {two}>"]
+    wgw = plan["op"].wg_waves or 16
+    return [f"etd2_stage<false, {kt}, {two}, {wgw}>", f"etd2_stage<true, {kt}, {two}, {wgw}>"]
 
 
 def fold_flops(plan):

@@ -51,7 +51,7 @@ typedef struct prh_operator {
   int32_t nspec;   /* species blocks */
   int32_t d;       /* energy bins per block (<= 64) */
   int32_t K;       /* right-hand sides */
-  int32_t ntasks;  /* multiple of PRH_WG_WAVES */
+  int32_t ntasks;  /* multiple of the workgroup waves (wg_waves, default PRH_WG_WAVES) */
   const int32_t* tasks;
   const int32_t* blocks;
   int64_t mvals_stride; /* doubles per window in mvals */
@@ -83,6 +83,11 @@ typedef struct prh_operator {
   int32_t nheavy;
   int32_t pad2_;
   const int32_t* wheavy;
+  /* Waves per workgroup of the narrow stage kernels: 0 or 16 (default), or 8
+   * (better at large K).  tasks are packed for it (ntasks a multiple); the
+   * adjoint and uneven kernels take 16 only. */
+  int32_t wg_waves;
+  int32_t pad3_;
 } prh_operator;
 
 #define PRH_F_PARTIAL_DIFF 1 /* continuous-loss stencil term on */

@@ -72,6 +72,8 @@ class PrhOperator(ctypes.Structure):
         ("nheavy", _i32),
         ("pad2_", _i32),
         ("wheavy", _p),
+        ("wg_waves", _i32),
+        ("pad3_", _i32),
     ]
 
 

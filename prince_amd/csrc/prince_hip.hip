@@ -120,6 +120,9 @@ __global__ void gather_rows_kernel(const double* __restrict__ src, const int64_t
 constexpr int SWG = PRH_WG_WAVES;     // waves per stage workgroup
 constexpr int STHREADS = SWG * WAVE;  // 1024
 
+// waves per workgroup of the narrow stage kernels for this operator
+inline int wg_waves_of(const prh_operator& op) { return op.wg_waves == 8 ? 8 : 16; }
+
 __device__ __forceinline__ double shfl_or_zero(double v, int src, int d) {
   const double r = __shfl(v, src < 0 ? 0 : (src > 63 ? 63 : src), WAVE);
   return (src >= 0 && src < d) ? r : 0.0;
@@ -306,8 +309,9 @@ __device__ __forceinline__ void reduce_parts(double (*red)[KT][WAVE], int wv, in
 // group before the next, so the coupling values of a few row groups and the
 // mother panels they read stay in that XCD's L2 (1.5x over walking all row
 // groups of one right-hand-side group first, measured at K = 512).
-__device__ __forceinline__ void wg_coords(int ntasks, int K, int kt, int& gx, int& gy) {
-  const int nx = ntasks / SWG;
+__device__ __forceinline__ void wg_coords(int ntasks, int K, int kt, int& gx, int& gy,
+                                          int wgw = SWG) {
+  const int nx = ntasks / wgw;
   const int ny = (K + kt - 1) / kt;
   const int L = blockIdx.x;
   if (ny % 8 == 0) {
@@ -405,11 +409,11 @@ __device__ __forceinline__ RowCoef row_coef_from(const RowIn& ri, double s, bool
 
 // step_local: step index (use_ctl = 0) or offset from step_ctl[0] (use_ctl = 1,
 // graph replay; steps at or beyond step_ctl[1] do nothing).
-template <bool STAGE2, int KT, bool TWO>
-__global__ void __launch_bounds__(STHREADS) etd2_stage(prh_operator op, prh_schedule sch,
-                                                       prh_state st, int step_local,
-                                                       int use_ctl) {
-  __shared__ double red[SWG][KT][WAVE];
+template <bool STAGE2, int KT, bool TWO, int WGW>
+__global__ void __launch_bounds__(WGW * WAVE) etd2_stage(prh_operator op, prh_schedule sch,
+                                                         prh_state st, int step_local,
+                                                         int use_ctl) {
+  __shared__ double red[WGW][KT][WAVE];
   int step = step_local;
   int ridx = -1;
   if (use_ctl) {
@@ -425,9 +429,9 @@ __global__ void __launch_bounds__(STHREADS) etd2_stage(prh_operator op, prh_sche
   const int K = op.K;
   const int d = op.d;
   int gx, gy;
-  wg_coords(op.ntasks, K, KT, gx, gy);
+  wg_coords(op.ntasks, K, KT, gx, gy, WGW);
   const int k0 = gy * KT;
-  const int* tk = op.tasks + 6 * ((int64_t)gx * SWG + wv);
+  const int* tk = op.tasks + 6 * ((int64_t)gx * WGW + wv);
   const int da = tk[0];
   const bool active = lane < d;
 
@@ -1433,15 +1437,19 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
   if (!op || !sch || !st) return set_error(std::string(who) + ": null argument");
   if (op->d <= 0 || op->d > WAVE) return set_error(std::string(who) + ": d must be in [1, 64]");
   if (op->K <= 0) return set_error(std::string(who) + ": K must be >= 1");
-  if (op->ntasks % SWG != 0)
-    return set_error(std::string(who) + ": ntasks must be a multiple of 16");
+  if (op->wg_waves != 0 && op->wg_waves != 8 && op->wg_waves != 16)
+    return set_error(std::string(who) + ": wg_waves must be 0 (16), 8 or 16");
+  if (op->ntasks % wg_waves_of(*op) != 0)
+    return set_error(std::string(who) + ": ntasks must be a multiple of the workgroup waves");
+  if (wg_waves_of(*op) != 16 && std::string(who).rfind("prh_adjoint", 0) == 0)
+    return set_error(std::string(who) + ": the adjoint kernels run 16-wave workgroups");
   if ((op->wide ? op->nwtasks : op->ntasks) <= 0)
     return set_error(std::string(who) + ": empty operator");
   if (!(op->kt == 1 || op->kt == 2 || op->kt == 4 || op->kt == 8 || op->kt == 16))
     return set_error(std::string(who) + ": kt must be 1, 2, 4, 8 or 16");
   if (op->kt == 16 && (op->mvals_e || std::string(who).rfind("prh_adjoint", 0) == 0 || st->rec_x))
     return set_error(std::string(who) + ": kt = 16 is for one-field forward stages without recording");
-  if ((int64_t)(op->ntasks / SWG) * ((op->K + op->kt - 1) / op->kt) > 0x7fffffff)
+  if ((int64_t)(op->ntasks / wg_waves_of(*op)) * ((op->K + op->kt - 1) / op->kt) > 0x7fffffff)
     return set_error(std::string(who) + ": grid too large");
   if (op->wide) {
     if (op->K % 64 != 0 || op->d != 64)
@@ -1461,35 +1469,45 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
   return validate_em(op, st, who);
 }
 
-template <bool S2, bool TWO>
-void launch_stage_two(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
-                      int step_local, int use_ctl, hipStream_t s) {
-  const int64_t nwg = (int64_t)(op.ntasks / SWG) * ((op.K + op.kt - 1) / op.kt);
+template <bool S2, bool TWO, int WGW>
+void launch_stage_wg(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
+                     int step_local, int use_ctl, hipStream_t s) {
+  const int64_t nwg = (int64_t)(op.ntasks / WGW) * ((op.K + op.kt - 1) / op.kt);
   const dim3 grid((unsigned)nwg);
+  constexpr int STH = WGW * WAVE;
   switch (op.kt) {
     case 1:
-      hipLaunchKernelGGL((etd2_stage<S2, 1, TWO>), grid, dim3(STHREADS), 0, s, op, sch, st,
+      hipLaunchKernelGGL((etd2_stage<S2, 1, TWO, WGW>), grid, dim3(STH), 0, s, op, sch, st,
                          step_local, use_ctl);
       break;
     case 2:
-      hipLaunchKernelGGL((etd2_stage<S2, 2, TWO>), grid, dim3(STHREADS), 0, s, op, sch, st,
+      hipLaunchKernelGGL((etd2_stage<S2, 2, TWO, WGW>), grid, dim3(STH), 0, s, op, sch, st,
                          step_local, use_ctl);
       break;
     case 4:
-      hipLaunchKernelGGL((etd2_stage<S2, 4, TWO>), grid, dim3(STHREADS), 0, s, op, sch, st,
+      hipLaunchKernelGGL((etd2_stage<S2, 4, TWO, WGW>), grid, dim3(STH), 0, s, op, sch, st,
                          step_local, use_ctl);
       break;
     case 16:
       if constexpr (!TWO) {
-        hipLaunchKernelGGL((etd2_stage<S2, 16, false>), grid, dim3(STHREADS), 0, s, op, sch, st,
+        hipLaunchKernelGGL((etd2_stage<S2, 16, false, WGW>), grid, dim3(STH), 0, s, op, sch, st,
                            step_local, use_ctl);
       }
       break;
     default:
-      hipLaunchKernelGGL((etd2_stage<S2, 8, TWO>), grid, dim3(STHREADS), 0, s, op, sch, st,
+      hipLaunchKernelGGL((etd2_stage<S2, 8, TWO, WGW>), grid, dim3(STH), 0, s, op, sch, st,
                          step_local, use_ctl);
       break;
   }
+}
+
+template <bool S2, bool TWO>
+void launch_stage_two(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
+                      int step_local, int use_ctl, hipStream_t s) {
+  if (wg_waves_of(op) == 8)
+    launch_stage_wg<S2, TWO, 8>(op, sch, st, step_local, use_ctl, s);
+  else
+    launch_stage_wg<S2, TWO, 16>(op, sch, st, step_local, use_ctl, s);
 }
 
 template <bool S2>
@@ -1763,6 +1781,7 @@ int32_t prh_uneven_step(const prh_operator* op, const prh_uneven* un, const prh_
   if (!(op->kt == 1 || op->kt == 2 || op->kt == 4))
     return set_error("prh_uneven_step: kt must be 1, 2 or 4");
   if (op->wide || op->mvals_e) return set_error("prh_uneven_step: narrow one-field operators only");
+  if (op->wg_waves == 8) return set_error("prh_uneven_step: the uneven kernel runs 16-wave workgroups");
   if (step < 0 || step >= un->nsteps) return set_error("prh_uneven_step: step outside the schedule");
   if (!un->h || !un->dldz || !un->hadia || !un->finj || !un->win || !un->rpair || !un->mrhs)
     return set_error("prh_uneven_step: null schedule buffer");

@@ -242,6 +242,15 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             return tuned
         return 24 if K == 1 else 96
 
+    def wg_waves_for(self, K, with_adjoint=False):
+        """Waves per stage workgroup: 8 from K = 64 on (more workgroups per CU:
+        151 vs 165 us per stage at K = 512), 16 below (rows split into up to 16
+        parts: 34 vs 43 us at K = 1) and for adjoint plans."""
+        tuned = getattr(self, "_tuned", {}).get(("wg_waves", K))
+        if tuned is not None:
+            return tuned
+        return 8 if (K >= 64 and not with_adjoint) else 16
+
     def kt_for(self, K):
         """Right-hand sides per wave: the largest of 8, 4, 2, 1 not above K."""
         tuned = getattr(self, "_tuned", {}).get(("kt", K))
@@ -527,7 +536,8 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             tasks = np.zeros((0, 6), dtype=np.int32)
             wtasks, wheavy, nwslots = build_wide_tasks(wl.fwd, self.wide_part_cost_for(K))
         else:
-            tasks = build_tasks(wl.fwd, self.part_cost_for(K), _lib.WG_WAVES)
+            wgw = self.wg_waves_for(K, with_adjoint)
+            tasks = build_tasks(wl.fwd, self.part_cost_for(K), wgw)
         kt = self.kt_for(K)
         col_sources = self._column_sources()
         has_src = any(len(c) for c in col_sources) and self.enable_injection_jacobian
@@ -582,7 +592,9 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             bufs["rpair_e"] = dv(rpair_e, np.float64)
             bufs["escale"] = dv(self._field_split[1], np.float64)
 
-        def make_op(tk, blk, dcf, wide_op=False):
+        fwd_wgw = 16 if wide else self.wg_waves_for(K, with_adjoint)
+
+        def make_op(tk, blk, dcf, wide_op=False, wgw=16):
             return _lib.PrhOperator(
                 nspec=nspec, d=d, K=K, ntasks=tk.shape[0],
                 tasks=tk.data_ptr() if tk.numel() else None,
@@ -600,9 +612,10 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
                 wtasks=bufs["wtasks"].data_ptr() if wide_op else None,
                 nheavy=int(wheavy.shape[0]) if wide_op else 0, pad2_=0,
                 wheavy=bufs["wheavy"].data_ptr() if wide_op else None,
+                wg_waves=wgw, pad3_=0,
             )
 
-        op = make_op(bufs["tasks"], bufs["blocks"], bufs["dcoef"], wide_op=wide)
+        op = make_op(bufs["tasks"], bufs["blocks"], bufs["dcoef"], wide_op=wide, wgw=fwd_wgw)
         opT = None
         if with_adjoint:
             tasks_t = build_tasks(wl.tr, self.part_cost_for(K), _lib.WG_WAVES)
