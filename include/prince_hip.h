@@ -87,7 +87,17 @@ typedef struct prh_operator {
    * (better at large K).  tasks are packed for it (ntasks a multiple); the
    * adjoint and uneven kernels take 16 only. */
   int32_t wg_waves;
-  int32_t pad3_;
+  /* Dense rows (narrow one-field stages only; ndense = 0: off).  Their blocks
+   * from the listed mothers are applied by a mother-major pass over chunks of
+   * dchunk species (ndchunk chunks): dense_tab [ndense][nspec][3] = {o_lo,
+   * nslot (0: no block), value offset}; dense_row [nspec] = dense index or -1;
+   * the chunk partials go to prh_state.dpart and are added in chunk order in
+   * those rows' epilogues.  The rows' tasks carry their remaining blocks. */
+  int32_t ndense;
+  int32_t dchunk;
+  int32_t ndchunk;
+  const int32_t* dense_row;
+  const int32_t* dense_tab;
 } prh_operator;
 
 #define PRH_F_PARTIAL_DIFF 1 /* continuous-loss stencil term on */
@@ -160,6 +170,7 @@ typedef struct prh_state {
   int32_t rec_count;
   double* wpart;     /* [nslots][d][K] partial row sums (wide layout) or NULL */
   const prh_em* em;  /* host pointer: EM sector stepped after stage 2, or NULL */
+  double* dpart;     /* [ndchunk][ndense][K][d] dense-row chunk partials, or NULL */
 } prh_state;
 
 /* Adjoint sweep buffers (all [nspec][K][d] unless noted). */

@@ -73,7 +73,11 @@ class PrhOperator(ctypes.Structure):
         ("pad2_", _i32),
         ("wheavy", _p),
         ("wg_waves", _i32),
-        ("pad3_", _i32),
+        ("ndense", _i32),
+        ("dchunk", _i32),
+        ("ndchunk", _i32),
+        ("dense_row", _p),
+        ("dense_tab", _p),
     ]
 
 
@@ -104,6 +108,7 @@ class PrhState(ctypes.Structure):
         ("rec_count", _i32),
         ("wpart", _p),
         ("em", _p),
+        ("dpart", _p),
     ]
 
 

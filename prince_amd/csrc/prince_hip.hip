@@ -456,6 +456,21 @@ __global__ void __launch_bounds__(WGW * WAVE) etd2_stage(prh_operator op, prh_sc
   }
   reduce_parts<KT>(red, wv, lane, tk[3], tk[4], acc);
   if (da < 0 || wv != tk[3]) return;
+  if (!TWO && op.ndense > 0) {
+    // dense rows: add the mother-chunk partial sums of dense_pass in chunk order
+    const int r = op.dense_row[da];
+    if (r >= 0) {
+#pragma unroll
+      for (int kk = 0; kk < KT; ++kk) {
+        const int k = k0 + kk;
+        if (k >= K || !active) continue;
+        double sum = 0.0;
+        for (int c = 0; c < op.ndchunk; ++c)
+          sum += st.dpart[(((int64_t)c * op.ndense + r) * K + k) * d + lane];
+        acc[kk] = acc[kk] + sum;
+      }
+    }
+  }
 
   // ---- epilogue: row block da, bins = lanes, KT right-hand sides
   const double dl = sch.dldz[step];
@@ -1432,6 +1447,97 @@ void launch_uneven(const prh_operator& op, const prh_uneven& un, const prh_state
   }
 }
 
+// ---------------------------------------------------------------------------
+// Dense rows (protons, He-4: a block from nearly every mother).  dense_pass
+// walks the mothers once per (mother chunk, KT right-hand sides): each mother
+// panel is loaded once and applied to every dense row that has a block from it
+// (diagonal or band, shuffles along the bins), and the chunk's row sums go to
+// dpart[chunk][row][rhs][bin].  The stage kernel adds the chunks in order in
+// the dense rows' epilogues; their other blocks stay in the ordinary tasks.
+// ---------------------------------------------------------------------------
+constexpr int DENSE_MAX = 2;
+
+template <int KT>
+__global__ void __launch_bounds__(256) dense_pass(prh_operator op, prh_schedule sch, prh_state st,
+                                                  int stage2, int step_local, int use_ctl) {
+  int step = step_local;
+  if (use_ctl) {
+    step = st.step_ctl[0] + step_local;
+    if (step >= st.step_ctl[1]) return;
+  }
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int K = op.K;
+  const int d = op.d;
+  const int ny = (K + KT - 1) / KT;
+  const int item = blockIdx.x * 4 + threadIdx.x / WAVE;
+  if (item >= op.ndchunk * ny) return;
+  const int c = item / ny;
+  const int k0 = (item % ny) * KT;
+  const bool active = lane < d;
+  const double* __restrict__ xin = stage2 ? st.a : st.x;
+  const int w = sch.win[step];
+  const double* __restrict__ mv = op.mvals + (int64_t)w * op.mvals_stride;
+  const int nr = op.ndense;
+  double acc[DENSE_MAX][KT];
+#pragma unroll
+  for (int r = 0; r < DENSE_MAX; ++r)
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) acc[r][kk] = 0.0;
+  const int mo0 = c * op.dchunk;
+  const int mo1 = min(mo0 + op.dchunk, op.nspec);
+  for (int mo = mo0; mo < mo1; ++mo) {
+    // every load of the mother (state panel and all diagonals of every dense
+    // row's block) is issued before any use
+    double y[KT];
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) {
+      const int k = k0 + kk;
+      y[kk] = (active && k < K) ? xin[((int64_t)mo * K + k) * d + lane] : 0.0;
+    }
+    double mr[DENSE_MAX][NSMAX];
+    int olo[DENSE_MAX], ns[DENSE_MAX];
+#pragma unroll
+    for (int r = 0; r < DENSE_MAX; ++r) {
+      const int* e = op.dense_tab + 3 * ((int64_t)(r < nr ? r : 0) * op.nspec + mo);
+      olo[r] = e[0];
+      ns[r] = r < nr ? e[1] : 0;
+      const int64_t voff = e[2];
+#pragma unroll
+      for (int s2 = 0; s2 < NSMAX; ++s2)
+        mr[r][s2] = (active && s2 < ns[r]) ? mv[voff + (int64_t)s2 * d + lane] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < DENSE_MAX; ++r) {
+#pragma unroll
+      for (int s2 = 0; s2 < NSMAX; ++s2) {
+        if (s2 >= ns[r]) break;  // uniform
+        const int src = lane + olo[r] + s2;
+#pragma unroll
+        for (int kk = 0; kk < KT; ++kk) acc[r][kk] += mr[r][s2] * shfl_or_zero(y[kk], src, d);
+      }
+    }
+  }
+  if (!active) return;
+#pragma unroll
+  for (int r = 0; r < DENSE_MAX; ++r) {
+    if (r >= nr) break;
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) {
+      const int k = k0 + kk;
+      if (k < K) st.dpart[(((int64_t)c * nr + r) * K + k) * d + lane] = acc[r][kk];
+    }
+  }
+}
+
+template <int KT>
+void launch_dense(const prh_operator& op, const prh_schedule& sch, const prh_state& st, int stage2,
+                  int step_local, int use_ctl, hipStream_t s) {
+  const int ny = (op.K + KT - 1) / KT;
+  const int64_t items = (int64_t)op.ndchunk * ny;
+  hipLaunchKernelGGL((dense_pass<KT>), dim3((unsigned)((items + 3) / 4)), dim3(256), 0, s, op, sch,
+                     st, stage2, step_local, use_ctl);
+}
+
 int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_state* st,
                  const char* who) {
   if (!op || !sch || !st) return set_error(std::string(who) + ": null argument");
@@ -1462,6 +1568,16 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
       return set_error(std::string(who) + ": split rows need wheavy and the partial buffer");
     if (op->nsrc > 4) return set_error(std::string(who) + ": the wide layout takes at most 4 sources per rhs");
     if (st->rec_x) return set_error(std::string(who) + ": the wide layout does not record steps");
+  }
+  if (op->ndense > 0) {
+    if (op->ndense > DENSE_MAX) return set_error(std::string(who) + ": at most 2 dense rows");
+    if (op->wide || op->mvals_e || op->kt > 8)
+      return set_error(std::string(who) + ": dense rows are for the narrow one-field stages (kt <= 8)");
+    if (!op->dense_row || !op->dense_tab || !st->dpart || op->dchunk < 1 || op->ndchunk < 1 ||
+        (int64_t)op->dchunk * op->ndchunk < op->nspec)
+      return set_error(std::string(who) + ": dense rows need dense_row, dense_tab, dpart and chunks covering the species");
+    if (std::string(who).rfind("prh_adjoint", 0) == 0)
+      return set_error(std::string(who) + ": the adjoint operator has no dense rows");
   }
   if ((op->mvals_e == nullptr) != (op->escale == nullptr) ||
       (op->mvals_e != nullptr && op->rpair_e == nullptr))
@@ -1517,10 +1633,19 @@ void launch_stage_kt(const prh_operator& op, const prh_schedule& sch, const prh_
     launch_wide<S2>(op, sch, st, step_local, use_ctl, s);
     return;
   }
-  if (op.mvals_e)
+  if (op.mvals_e) {
     launch_stage_two<S2, true>(op, sch, st, step_local, use_ctl, s);
-  else
-    launch_stage_two<S2, false>(op, sch, st, step_local, use_ctl, s);
+    return;
+  }
+  if (op.ndense > 0) {
+    switch (op.kt) {
+      case 1: launch_dense<1>(op, sch, st, S2, step_local, use_ctl, s); break;
+      case 2: launch_dense<2>(op, sch, st, S2, step_local, use_ctl, s); break;
+      case 4: launch_dense<4>(op, sch, st, S2, step_local, use_ctl, s); break;
+      default: launch_dense<8>(op, sch, st, S2, step_local, use_ctl, s); break;
+    }
+  }
+  launch_stage_two<S2, false>(op, sch, st, step_local, use_ctl, s);
 }
 
 template <bool S2, bool TWO>

@@ -191,6 +191,37 @@ def build_tasks(layout: BlockLayout, part_cost=96, wg_waves=16):
     return np.asarray(tasks, dtype=np.int32).reshape(-1, 6)
 
 
+def dense_split(blocks, brow_ptr, nspec, min_blocks=100, max_rows=2, max_slots=8):
+    """Rows with at least ``min_blocks`` blocks (at most ``max_rows`` of them):
+    their blocks of <= ``max_slots`` diagonals move to the mother-major dense
+    pass.
+
+    Returns None, or ``(rows, row_of, tab, blocks_rest, brow_rest)``: the dense
+    rows, ``row_of[da]`` = dense index or -1, ``tab[r][mo] = (o_lo, nslot,
+    value offset)`` (nslot 0: no block) and the block list / row pointer of
+    everything the ordinary tasks still carry (same order, same offsets).
+    """
+    counts = np.diff(np.asarray(brow_ptr, dtype=np.int64))
+    order = np.argsort(-counts, kind="stable")[:max_rows]
+    rows = sorted(int(r) for r in order if counts[r] >= min_blocks)
+    if not rows:
+        return None
+    blocks = np.asarray(blocks)
+    keep = np.ones(blocks.shape[0], dtype=bool)
+    tab = np.zeros((len(rows), nspec, 3), dtype=np.int32)
+    row_of = np.full(nspec, -1, dtype=np.int32)
+    for ri, da in enumerate(rows):
+        row_of[da] = ri
+        for b in range(int(brow_ptr[da]), int(brow_ptr[da + 1])):
+            mo, olo, ns, voff = (int(v) for v in blocks[b])
+            if ns <= max_slots:
+                tab[ri, mo] = (olo, ns, voff)
+                keep[b] = False
+    per_row = np.array([keep[int(brow_ptr[i]):int(brow_ptr[i + 1])].sum() for i in range(nspec)])
+    brow_rest = np.concatenate([[0], np.cumsum(per_row)]).astype(np.int64)
+    return rows, row_of, tab, blocks[keep], brow_rest
+
+
 def wide_block_cost(blocks):
     """Cost of a block in the wide stage kernel (lanes = right-hand sides):
     one state load per bin plus one FMA per bin and diagonal."""

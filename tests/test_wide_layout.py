@@ -128,3 +128,46 @@ def test_nuclear_layout_has_only_wide_shapes():
     bad = lay.blocks.copy()
     bad[0, 1:3] = (2, 3)
     assert not wide_shapes_ok(bad)
+
+
+def test_dense_split_keeps_the_product():
+    """Dense rows (protons, He-4) moved to the mother-major pass: the blocks left
+    in the tasks plus the dense table give the same off-diagonal product."""
+    from prince_amd.solvers.layout import dense_split
+
+    run = workloads.nuclear_run()
+    lay, m0, M = _layout(run)
+    wl = WindowLayout(lay, None)
+    mv = _window_values(wl, m0)
+    nspec, d = lay.nspec, lay.d
+    out = dense_split(wl.blocks, lay.brow_ptr, nspec)
+    assert out is not None
+    rows, row_of, tab, rest, brow_rest = out
+    pdg = [run.spec_man.species_refs[r].pdgid for r in rows]
+    assert 2212 in pdg and 1000020040 in pdg
+    assert rest.shape[0] + int((tab[:, :, 1] > 0).sum()) == wl.blocks.shape[0]
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((nspec, d))
+
+    def apply(b, acc_row):
+        mo, olo, ns, voff = (int(v) for v in b)
+        for s2 in range(ns):
+            for i in range(d):
+                j = i + olo + s2
+                if 0 <= j < d:
+                    acc_row[i] += mv[voff + s2 * d + i] * X[mo, j]
+
+    got = np.zeros((nspec, d))
+    for da in range(nspec):
+        for b in rest[brow_rest[da]:brow_rest[da + 1]]:
+            apply(b, got[da])
+    for ri, da in enumerate(rows):
+        for mo in range(nspec):
+            olo, ns, voff = tab[ri, mo]
+            if ns > 0:
+                apply((mo, olo, ns, voff), got[da])
+    Moff = M - sp.diags(M.diagonal())
+    ref = (Moff @ X.reshape(-1)).reshape(nspec, d)
+    scale = (np.abs(Moff) @ np.abs(X.reshape(-1))).reshape(nspec, d)
+    assert np.all(np.abs(got - ref) <= 1e-13 * np.maximum(scale, 1e-300))
+    assert row_of[rows[0]] == 0 and (row_of >= 0).sum() == len(rows)
