@@ -54,6 +54,7 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true")
     p.add_argument("--adjoint-batch", type=int, default=64)
     p.add_argument("--no-uneven", action="store_true")
+    p.add_argument("--no-autotune", action="store_true")
     p.add_argument("--uneven-batch", type=int, default=1024)
     p.add_argument("--z-max", type=float, default=3.0,
                    help="initial redshift (3.0 is the benchmark; smaller values shorten profiler runs)")
@@ -120,6 +121,14 @@ def stage_bytes(plan, n, K):
     matrix = 8.0 * (lay.fwd.zslots + n) + 16.0 * lay.blocks.shape[0]
     src = plan["bufs"]["src_grid"].numel() * 8.0
     return state + matrix + src
+
+
+def _cfg_str(cfg):
+    """Readable form of an autotuner configuration."""
+    parts = []
+    for k, v in cfg.items():
+        parts.append(f"{k if isinstance(k, str) else k[0]}={v}")
+    return ",".join(parts)
 
 
 def stage_kernel_names(plan):
@@ -309,6 +318,17 @@ def main():
     for k in range(K):
         multi[k].add_source_class(srcs[k])
     t0 = time.perf_counter()
+    tuning = None
+    if not args.no_autotune:
+        table = multi.autotune(1e-3, steps=30)
+        tuning = {
+            "seconds": round(time.perf_counter() - t0, 1),
+            "chosen": _cfg_str(table[0][1]),
+            "table_us_per_step": [[round(t * 1e6, 1), _cfg_str(c)] for t, c in table],
+        }
+        log(f"autotune ({tuning['seconds']}s): {tuning['chosen']} "
+            f"{table[0][0]*1e6:.1f} us/step; slowest {table[-1][0]*1e6:.1f}")
+    t0 = time.perf_counter()
     bplan = multi.prepare(1e-3)
     log(f"batched plan K={K} prepared in {time.perf_counter() - t0:.1f}s")
     for _ in range(max(0, args.warmup)):
@@ -445,6 +465,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,
+        "autotune": tuning,
     }
     _ = plain_state
     print(json.dumps(out))

@@ -777,6 +777,74 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         self._launch_steps(plan, k_begin, k_end)
         return plan["x"]
 
+    # ---------------------------------------------------------------- autotuner
+    def tuning_candidates(self, K):
+        """Stage configurations the autotuner times for K right-hand sides:
+        right-hand sides per wave (kt), part budget (block cost per wave part,
+        i.e. rows per wave), waves per workgroup, and the wide layout when it
+        applies."""
+        kts = [kt for kt in (8, 4, 2, 1) if kt <= K][:2]
+        out = []
+        for kt in kts:
+            for pc in (48, 96) if K > 1 else (12, 24):
+                for wgw in ((8, 16) if K >= 8 else (16,)):
+                    out.append({"layout": "narrow", ("kt", K): kt, ("part_cost", K): pc,
+                                ("wg_waves", K): wgw})
+        saved = getattr(self, "_tuned", None)
+        self._tuned = {"layout": "wide"}
+        try:
+            wide_ok = self._use_wide()
+        except ValueError:
+            wide_ok = False
+        finally:
+            if saved is None:
+                del self._tuned
+            else:
+                self._tuned = saved
+        if wide_ok:
+            out.append({"layout": "wide", ("wide_part_cost", K): 96})
+        return out
+
+    def autotune(self, dz=1e-3, steps=30, candidates=None, repeats=2):
+        """Time each candidate stage configuration on this operator for ``steps``
+        ETD2 steps (HIP events on the launch stream, after one untimed pass) and
+        keep the fastest in ``self._tuned``.  Returns the timing table
+        [(seconds per step, config)], fastest first.  Results do not depend on
+        the choice beyond summation order (every candidate is parity-tested)."""
+        torch = _torch()
+        K = self._K
+        cands = candidates if candidates is not None else self.tuning_candidates(K)
+        table = []
+        for cfg in cands:
+            self._tuned = dict(cfg)
+            try:
+                plan = self.prepare(dz)
+            except (ValueError, NotImplementedError):
+                continue
+            n = min(steps, plan["nsteps"])
+            plan["x"].zero_()
+            self.fold_windows(plan)
+            self._launch_steps(plan, 0, n, use_graph=False)
+            best = None
+            for _ in range(repeats):
+                plan["x"].zero_()
+                ev0 = torch.cuda.Event(enable_timing=True)
+                ev1 = torch.cuda.Event(enable_timing=True)
+                ev0.record()
+                self._launch_steps(plan, 0, n, use_graph=False)
+                ev1.record()
+                torch.cuda.synchronize(self.device)
+                t = ev0.elapsed_time(ev1) * 1e-3 / n
+                best = t if best is None else min(best, t)
+            table.append((best, cfg))
+        self.close()
+        if not table:
+            raise RuntimeError("no stage configuration could be prepared")
+        table.sort(key=lambda r: r[0])
+        self._tuned = dict(table[0][1])
+        self.tuning_table = table
+        return table
+
     def _launch_steps(self, plan, k_begin, k_end, use_graph=True):
         if use_graph and plan["graph"] is not None:
             plan["graph"].run(k_begin, k_end, plan["stream"])

@@ -181,3 +181,19 @@ def test_dense_rows_pass_matches_oracle_and_plain_tasks():
         worst, worst_oo = max(worst, dev), max(worst_oo, dev2)
     print(f"dense pass: vs oracle {worst:.3e}, vs ordinary tasks {worst_oo:.3e}")
     assert worst <= TOL and worst_oo <= TOL
+
+
+def test_autotuner_picks_a_candidate_and_keeps_results():
+    run = workloads.nuclear_run()
+    K = 64
+    srcs = workloads.batch_sources(run, K, seed=9)
+    g = _multi(run, 0.2, K, srcs)
+    table = g.autotune(1e-3, steps=5, repeats=1)
+    assert len(table) >= 4 and all(t > 0 for t, _ in table)
+    assert g._tuned == table[0][1]
+    g.solve(dz=1e-3)
+    ref = _multi(run, 0.2, K, srcs)
+    ref.solve(dz=1e-3)
+    for k in range(K):
+        dev, nb = rel_dev(g.state[:, k], ref.state[:, k])
+        assert nb > 100 and dev <= TOL, (k, dev, table[0][1])
