@@ -137,6 +137,9 @@ typedef struct prh_em {
   double* wbh;          /* [d][K] scratch: Bethe-Heitler weights of the step */
 } prh_em;
 
+/* Opaque handle of a captured step graph (prh_graph_create, prh_uneven_graph_create). */
+typedef struct prh_graph prh_graph;
+
 /* Uneven batch: every right-hand side k has its own z grid (its own z_max),
  * refresh windows and coupling values.  Per-step tables are [nsteps][K];
  * win < 0 marks a finished column (its state is left as it is).  mrhs holds
@@ -154,6 +157,20 @@ typedef struct prh_uneven {
   const double* rpair;   /* [nwin][d] pair-production rate of each column window */
   const double* mrhs;    /* [K][mvals_stride] current window values of each column */
 } prh_uneven;
+
+/* Per-step re-fold lists of the uneven batch for graph capture: at step t the
+ * columns out_rows[ref_off[t] .. ref_off[t+1]) are re-folded from the photon
+ * rows in_rows[...] (device arrays); ref_off is a HOST array of nsteps + 1. */
+typedef struct prh_refold {
+  const double* kern;      /* [nrows][dph] window-layout fold kernel */
+  int64_t nrows;
+  int32_t dph;
+  int32_t enabled;         /* 0: photo-hadronic coupling off, no re-folds */
+  const double* phot;      /* [nwin][dph] photon vectors of all column windows */
+  const int32_t* in_rows;  /* device */
+  const int32_t* out_rows; /* device */
+  const int32_t* ref_off;  /* host, [nsteps + 1] */
+} prh_refold;
 
 typedef struct prh_state {
   double* x;         /* [nspec][K][d] solution, updated in place */
@@ -269,6 +286,14 @@ int32_t prh_fold_rows(const double* kern, int64_t nrows, int32_t dph, const doub
 int32_t prh_uneven_step(const prh_operator* op, const prh_uneven* un, const prh_state* st,
                         int32_t step, void* stream);
 
+/* Capture steps [k_begin, k_end) of the uneven batch -- each step's column
+ * re-fold (prh_fold_rows) and both stages (prh_uneven_step), steps baked in --
+ * as one graph; run it with prh_graph_run(g, k_begin, k_end, stream) and free
+ * it with prh_graph_destroy.  The 64-step chunks of the uneven solve. */
+int32_t prh_uneven_graph_create(const prh_operator* op, const prh_uneven* un, const prh_state* st,
+                                const prh_refold* rf, int32_t k_begin, int32_t k_end,
+                                prh_graph** out);
+
 /* u = h phi2 lam for step `step`, before the first prh_adjoint_step of a sweep. */
 int32_t prh_adjoint_seed(const prh_operator* opT, const prh_schedule* sch, const prh_adjoint* adj,
                          int32_t step, void* stream);
@@ -278,7 +303,6 @@ int32_t prh_adjoint_seed(const prh_operator* opT, const prh_schedule* sch, const
  * st->step_ctl, so one instantiated graph replays every chunk of a solve.
  * Replaces the per-window CUDA-graph capture/replay of
  * src/prince_cr/solvers/propagation.py:2186-2337. */
-typedef struct prh_graph prh_graph;
 int32_t prh_graph_create(const prh_operator* op, const prh_schedule* sch, const prh_state* st,
                          int32_t chunk, prh_graph** out);
 int32_t prh_graph_run(prh_graph* g, int32_t k_begin, int32_t k_end, void* stream);

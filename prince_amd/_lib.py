@@ -29,6 +29,7 @@ EXPORTED = (
     "prh_field_tables",
     "prh_fold_rows",
     "prh_uneven_step",
+    "prh_uneven_graph_create",
 )
 
 WG_WAVES = 16
@@ -147,6 +148,19 @@ class PrhUneven(ctypes.Structure):
     ]
 
 
+class PrhRefold(ctypes.Structure):
+    _fields_ = [
+        ("kern", _p),
+        ("nrows", _i64),
+        ("dph", _i32),
+        ("enabled", _i32),
+        ("phot", _p),
+        ("in_rows", _p),
+        ("out_rows", _p),
+        ("ref_off", _p),
+    ]
+
+
 class PrhAdjoint(ctypes.Structure):
     _fields_ = [
         ("lam", _p),
@@ -226,6 +240,16 @@ def load():
         _i32,
         _p,
     ]
+    lib.prh_uneven_graph_create.restype = _i32
+    lib.prh_uneven_graph_create.argtypes = [
+        ctypes.POINTER(PrhOperator),
+        ctypes.POINTER(PrhUneven),
+        ctypes.POINTER(PrhState),
+        ctypes.POINTER(PrhRefold),
+        _i32,
+        _i32,
+        ctypes.POINTER(_p),
+    ]
     lib.prh_em_step.restype = _i32
     lib.prh_em_step.argtypes = [
         ctypes.POINTER(PrhOperator),
@@ -296,6 +320,25 @@ class StepGraph:
             self.close()
         except Exception:
             pass
+
+
+class UnevenChunkGraph(StepGraph):
+    """Owned handle of a captured uneven-batch chunk (steps baked in)."""
+
+    def __init__(self, op, un, st, rf, k_begin, k_end):
+        lib = load()
+        h = _p()
+        check(
+            lib.prh_uneven_graph_create(ctypes.byref(op), ctypes.byref(un), ctypes.byref(st),
+                                        ctypes.byref(rf), int(k_begin), int(k_end), ctypes.byref(h)),
+            "uneven chunk capture",
+        )
+        self._h = h
+        self.chunk = int(k_end - k_begin)
+        self.k_begin, self.k_end = int(k_begin), int(k_end)
+
+    def launch(self, stream):
+        self.run(self.k_begin, self.k_end, stream)
 
 
 def check(status, what):

@@ -1206,6 +1206,24 @@ void launch_em(const prh_operator& op, const prh_schedule& sch, const prh_state&
                      step_local, use_ctl);
 }
 
+int32_t check_uneven(const prh_operator* op, const prh_uneven* un, const prh_state* st,
+                     const char* who) {
+  const std::string w(who);
+  if (!op || !un || !st) return set_error(w + ": null argument");
+  if (op->d <= 0 || op->d > WAVE) return set_error(w + ": d must be in [1, 64]");
+  if (op->K <= 0) return set_error(w + ": K must be >= 1");
+  if (op->ntasks <= 0 || op->ntasks % SWG != 0)
+    return set_error(w + ": ntasks must be a positive multiple of 16");
+  if (!(op->kt == 1 || op->kt == 2 || op->kt == 4)) return set_error(w + ": kt must be 1, 2 or 4");
+  if (op->wide || op->mvals_e) return set_error(w + ": narrow one-field operators only");
+  if (op->wg_waves == 8) return set_error(w + ": the uneven kernel runs 16-wave workgroups");
+  if (op->ndense > 0) return set_error(w + ": the uneven kernel has no dense-row pass");
+  if (!un->h || !un->dldz || !un->hadia || !un->finj || !un->win || !un->rpair || !un->mrhs)
+    return set_error(w + ": null schedule buffer");
+  if ((op->flags & PRH_F_INJECTION) && !un->src_coef) return set_error(w + ": injection needs src_coef");
+  return 0;
+}
+
 int32_t validate_em(const prh_operator* op, const prh_state* st, const char* who) {
   if (!st->em) return 0;
   const prh_em* em = st->em;
@@ -1674,6 +1692,7 @@ void launch_adjoint_two(const prh_operator& op, const prh_schedule& sch, const p
 struct prh_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  int32_t baked_first = -1;  // >= 0: steps baked in at capture (uneven chunks)
   int32_t chunk = 0;
   int32_t nsteps = 0;
   int32_t* step_ctl = nullptr;
@@ -1845,6 +1864,13 @@ int32_t prh_graph_create(const prh_operator* op, const prh_schedule* sch, const 
 
 int32_t prh_graph_run(prh_graph* g, int32_t k_begin, int32_t k_end, void* stream) {
   if (!g || !g->exec) return set_error("prh_graph_run: invalid graph");
+  if (g->baked_first >= 0) {
+    if (k_begin != g->baked_first || k_end != g->baked_first + g->chunk)
+      return set_error("prh_graph_run: an uneven chunk graph runs exactly its captured steps");
+    hipError_t err = hipGraphLaunch(g->exec, (hipStream_t)stream);
+    if (err != hipSuccess) return set_error(std::string("prh_graph_run: ") + hipGetErrorString(err));
+    return check_hip(hipGetLastError(), "prh_graph_run");
+  }
   if (k_begin < 0 || k_end < k_begin || k_end > g->nsteps)
     return set_error("prh_graph_run: step range outside the schedule");
   if (k_begin == k_end) return 0;
@@ -1896,22 +1922,56 @@ int32_t prh_fold_rows(const double* kern, int64_t nrows, int32_t dph, const doub
   return check_hip(hipGetLastError(), "prh_fold_rows launch");
 }
 
+int32_t prh_uneven_graph_create(const prh_operator* op, const prh_uneven* un, const prh_state* st,
+                                const prh_refold* rf, int32_t k_begin, int32_t k_end,
+                                prh_graph** out) {
+  if (!out) return set_error("prh_uneven_graph_create: null output");
+  *out = nullptr;
+  if (!rf || !rf->ref_off) return set_error("prh_uneven_graph_create: null re-fold lists");
+  if (k_begin < 0 || k_end <= k_begin || k_end > (un ? un->nsteps : 0))
+    return set_error("prh_uneven_graph_create: step range outside the schedule");
+  if (int32_t e = check_uneven(op, un, st, "prh_uneven_graph_create")) return e;
+  for (int32_t t = k_begin; t < k_end; ++t)
+    if (rf->ref_off[t + 1] < rf->ref_off[t])
+      return set_error("prh_uneven_graph_create: re-fold offsets must not decrease");
+  hipStream_t cap = nullptr;
+  if (int32_t e = check_hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking),
+                            "prh_uneven_graph_create stream"))
+    return e;
+  prh_graph* g = new prh_graph();
+  g->baked_first = k_begin;
+  g->chunk = k_end - k_begin;
+  g->nsteps = un->nsteps;
+  int32_t status = 0;
+  hipError_t err = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (err == hipSuccess) {
+    for (int32_t t = k_begin; t < k_end && status == 0; ++t) {
+      const int32_t n = rf->ref_off[t + 1] - rf->ref_off[t];
+      if (rf->enabled && n > 0)
+        status = prh_fold_rows(rf->kern, rf->nrows, rf->dph, rf->phot, rf->in_rows + rf->ref_off[t],
+                               rf->out_rows + rf->ref_off[t], n, const_cast<double*>(un->mrhs),
+                               op->mvals_stride, cap);
+      if (status == 0) status = prh_uneven_step(op, un, st, t, cap);
+    }
+    err = hipStreamEndCapture(cap, &g->graph);
+  }
+  if (err == hipSuccess && status == 0) err = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  hipStreamDestroy(cap);
+  if (err != hipSuccess || status != 0) {
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
+    if (status != 0) return status;
+    return set_error(std::string("prh_uneven_graph_create: ") + hipGetErrorString(err));
+  }
+  *out = g;
+  return 0;
+}
+
 int32_t prh_uneven_step(const prh_operator* op, const prh_uneven* un, const prh_state* st,
                         int32_t step, void* stream) {
-  if (!op || !un || !st) return set_error("prh_uneven_step: null argument");
-  if (op->d <= 0 || op->d > WAVE) return set_error("prh_uneven_step: d must be in [1, 64]");
-  if (op->K <= 0) return set_error("prh_uneven_step: K must be >= 1");
-  if (op->ntasks <= 0 || op->ntasks % SWG != 0)
-    return set_error("prh_uneven_step: ntasks must be a positive multiple of 16");
-  if (!(op->kt == 1 || op->kt == 2 || op->kt == 4))
-    return set_error("prh_uneven_step: kt must be 1, 2 or 4");
-  if (op->wide || op->mvals_e) return set_error("prh_uneven_step: narrow one-field operators only");
-  if (op->wg_waves == 8) return set_error("prh_uneven_step: the uneven kernel runs 16-wave workgroups");
+  if (int32_t e = check_uneven(op, un, st, "prh_uneven_step")) return e;
   if (step < 0 || step >= un->nsteps) return set_error("prh_uneven_step: step outside the schedule");
-  if (!un->h || !un->dldz || !un->hadia || !un->finj || !un->win || !un->rpair || !un->mrhs)
-    return set_error("prh_uneven_step: null schedule buffer");
-  if ((op->flags & PRH_F_INJECTION) && !un->src_coef)
-    return set_error("prh_uneven_step: injection needs src_coef");
   hipStream_t s = (hipStream_t)stream;
   launch_uneven<false>(*op, *un, *st, step, s);
   launch_uneven<true>(*op, *un, *st, step, s);

@@ -266,8 +266,38 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
         self.timings["setup_host_s"] = time.perf_counter() - t0
         return self._plan
 
+    def _chunk_graphs(self, plan):
+        """Capture the solve as 64-step chunk graphs (re-folds and stages of every
+        step baked in); built once per plan."""
+        if plan.get("graphs") is not None:
+            return plan["graphs"]
+        b = plan["bufs"]
+        ro = np.ascontiguousarray(plan["ref_off"], dtype=np.int32)
+        plan["ref_off_i32"] = ro
+        rf = _lib.PrhRefold(
+            kern=b["B_sel"].data_ptr(), nrows=plan["stride"], dph=plan["dph"],
+            enabled=1 if self.enable_photohad_losses else 0, phot=b["phot"].data_ptr(),
+            in_rows=b["ref_in"].data_ptr(), out_rows=b["ref_out"].data_ptr(),
+            ref_off=ro.ctypes.data,
+        )
+        plan["refold"] = rf
+        T, C = plan["nsteps"], self.graph_chunk
+        plan["graphs"] = [
+            _lib.UnevenChunkGraph(plan["op"], plan["un"], plan["st"], rf, c0, min(c0 + C, T))
+            for c0 in range(0, T, C)
+        ]
+        return plan["graphs"]
+
+    def close(self):
+        if self._plan is not None:
+            for g in self._plan.get("graphs") or []:
+                g.close()
+            self._plan["graphs"] = None
+        super().close()
+
     def execute(self, plan=None, k_begin=0, k_end=None):
-        """Device part of the batch: zero state, then per step the re-folds and both stages."""
+        """Device part of the batch: zero state, then per step the re-folds and both
+        stages (replayed as 64-step chunk graphs when the whole range is run)."""
         plan = plan if plan is not None else self._plan
         b = plan["bufs"]
         lib = self._lib
@@ -275,6 +305,13 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
         k_end = plan["nsteps"] if k_end is None else k_end
         if k_begin == 0:
             plan["x"].zero_()
+        C = self.graph_chunk
+        if (getattr(self.backend, "use_hip_graphs", True) and k_begin % C == 0
+                and (k_end % C == 0 or k_end == plan["nsteps"])):
+            graphs = self._chunk_graphs(plan)
+            for g in graphs[k_begin // C: -(-k_end // C)]:
+                g.launch(stream)
+            return plan["x"]
         ro = plan["ref_off"]
         isz = ctypes.sizeof(ctypes.c_int32)
         for t in range(k_begin, k_end):

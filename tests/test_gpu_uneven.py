@@ -59,3 +59,22 @@ def test_uneven_batch_equal_columns_equal_plain_solve():
     for k in range(K):
         dev, nb = rel_dev(g.state[:, k], p.state)
         assert nb > 100 and dev <= TOL, (k, dev)
+
+
+def test_uneven_chunk_graphs_equal_direct_launches():
+    run = workloads.nuclear_run()
+    K = 5
+    rng = np.random.default_rng(2)
+    z_max = rng.uniform(0.1, 0.2, size=K)
+    srcs = workloads.batch_sources(run, K, seed=3)
+    states = []
+    for graphs in (True, False):
+        run.backend.use_hip_graphs = graphs
+        g = UnevenBatchSolverETD2(z_max, run, ebl_scale=rng.uniform(0.6, 1.4, size=K) * 0 + 1.0)
+        for k in range(K):
+            g[k].add_source_class(srcs[k])
+        g.solve(dz=1e-3)
+        assert (g._plan.get("graphs") is not None) == graphs
+        states.append(g.state.copy())
+    run.backend.use_hip_graphs = True
+    assert np.array_equal(states[0], states[1])
