@@ -1,0 +1,54 @@
+"""Write the golden fixtures of this directory (run from the repository root:
+``python tests/golden/make_golden.py``).
+
+The reference cannot run here (its import fetches a database), so these are
+regression fixtures of the FP64 CPU oracle, whose functions are pinned by
+closed forms in ``tests/test_oracle_pinning.py``, ``tests/test_model.py`` and
+``tests/test_cascade_model.py``.  Each file holds the inputs that define the
+case and the expected final state:
+
+* ``etd2_proton_cmb_z0.05.npz``: protons on the CMB, SimpleSource
+  (E^-2, 10^11.5 GeV), z 0.05 -> 0, dz 1e-3;
+* ``etd2_nuclear_si28_z0.1.npz``: the A <= 56 run (CMB + EBL seed 0), pure Si-28
+  source with the broken (1+z)^3 evolution, z 0.1 -> 0;
+* ``etd2_proton_em_z0.05.npz``: protons on the CMB with the EM cascade, z 0.05 -> 0
+  (nuclear state and photon spectrum).
+"""
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from oracle.etd2_cpu import OracleETD2  # noqa: E402
+from prince_amd import workloads  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def case(name, run, src, zi, **extra):
+    o = OracleETD2(zi, 0.0, run)
+    o.add_source_class(src)
+    o.solve(1e-3)
+    out = {"initial_z": zi, "final_z": 0.0, "dz": 1e-3, "state": o.state,
+           "egrid": run.cr_grid.grid, "pdg": np.array(run.spec_man.known_species),
+           "refresh_zs": np.array(o.refresh_zs)}
+    if o.em:
+        out["photons"] = o.em_state[22]
+        out["em_egrid"] = run.em_grid.grid
+    out.update(extra)
+    np.savez_compressed(os.path.join(HERE, name), **out)
+    print(name, o.state.shape, len(o.refresh_zs))
+
+
+if __name__ == "__main__":
+    run = workloads.proton_run()
+    case("etd2_proton_cmb_z0.05.npz", run, workloads.proton_source(run), 0.05)
+    run = workloads.nuclear_run()
+    case("etd2_nuclear_si28_z0.1.npz", run, workloads.silicon_source(run), 0.1)
+    run = workloads.proton_em_run()
+    case("etd2_proton_em_z0.05.npz", run, workloads.proton_source(run), 0.05)

@@ -197,3 +197,23 @@ def test_autotuner_picks_a_candidate_and_keeps_results():
     for k in range(K):
         dev, nb = rel_dev(g.state[:, k], ref.state[:, k])
         assert nb > 100 and dev <= TOL, (k, dev, table[0][1])
+
+
+@pytest.mark.parametrize("name", ["etd2_proton_cmb_z0.05.npz", "etd2_nuclear_si28_z0.1.npz",
+                                  "etd2_proton_em_z0.05.npz"])
+def test_gpu_against_golden_fixture(name):
+    from .test_golden import CASES, load
+
+    _, run_fn, src_fn = next(c for c in CASES if c[0] == name)
+    g = load(name)
+    run = run_fn()
+    s = UHECRPropagationSolverETD2(float(g["initial_z"]), float(g["final_z"]), run)
+    s.add_source_class(src_fn(run))
+    s.solve(dz=float(g["dz"]))
+    n = run.dim_states
+    dev, nb = rel_dev(s.state[:n], g["state"])
+    print(f"{name}: {nb} bins, max rel dev {dev:.3e}")
+    assert nb > 30 and dev <= TOL
+    if "photons" in g:
+        dev, nb = rel_dev(s.state[run.em_slices()[22]], g["photons"])
+        assert nb > 30 and dev <= TOL
