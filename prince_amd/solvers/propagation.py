@@ -603,7 +603,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             "fphi": torch.zeros(sshape, dtype=torch.float64, device=dev),
             "step_ctl": torch.zeros(2, dtype=torch.int32, device=dev),
         }
-        DENSE_CHUNK = 16
+        DENSE_CHUNK = getattr(self, "_tuned", {}).get(("dense_chunk", K), 16)
         ndchunk = -(-nspec // DENSE_CHUNK)
         if dense is not None:
             bufs["dense_row"] = dv(dense[1], np.int32)

@@ -29,7 +29,7 @@ from ..model import config as mconfig
 from ..model.cosmology import H
 from ..model.photonfields import PLANCK_PREF, CMBPhotonSpectrum, CombinedPhotonField, SyntheticEBL
 from ..model.units import PRINCE_UNITS
-from .layout import build_tasks, fd_tables
+from .layout import build_tasks
 from .propagation import (
     MultiRHSPropagationSolverETD2,
     _MultiRHSETD2SolverHIP,
@@ -103,7 +103,7 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
         """Photon vectors [nwin][dph] and pair rates [nwin][d] of all column windows."""
         torch = _torch()
         run = self.prince_run
-        cmb, ebl = _field_parts(run.photon_field)
+        _, ebl = _field_parts(run.photon_field)
         ir = self.had_int_rates
         pair = self.pair_loss_rates_grid
         stream = torch.cuda.current_stream(dev).cuda_stream
