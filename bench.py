@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true")
     p.add_argument("--adjoint-batch", type=int, default=64)
     p.add_argument("--no-uneven", action="store_true")
+    p.add_argument("--no-em", action="store_true")
+    p.add_argument("--em-batch", type=int, default=512)
     p.add_argument("--no-autotune", action="store_true")
     p.add_argument("--uneven-batch", type=int, default=1024)
     p.add_argument("--z-max", type=float, default=3.0,
@@ -293,6 +295,37 @@ def uneven_leg(run, K, torch, log, seed=0):
     }
 
 
+def em_leg(K, torch, log, seed=0):
+    """Nuclear run with the co-evolved EM cascade (photons, e+-), K draws, z 3 -> 0:
+    host setup (cascade transfers of the 288 windows, Bethe-Heitler shapes) and one
+    timed device solve; the photon spectrum's energy is reported."""
+    from prince_amd import workloads
+    from prince_amd.solvers.propagation import MultiRHSPropagationSolverETD2
+
+    run = workloads.nuclear_em_run()
+    srcs = workloads.batch_sources(run, K, seed=seed)
+    g = MultiRHSPropagationSolverETD2(3.0, 0.0, run, K=K)
+    for k in range(K):
+        g[k].add_source_class(srcs[k])
+    t0 = time.perf_counter()
+    plan = g.prepare(1e-3)
+    t_setup = time.perf_counter() - t0
+    g.execute(plan)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.execute(plan)
+    torch.cuda.synchronize()
+    t_run = time.perf_counter() - t0
+    gam = plan["em_bufs"]["gam"].cpu().numpy()
+    E = run.em_grid.grid
+    e_ph = float(np.mean((E * np.gradient(E))[:, None] * gam, axis=0).sum())
+    log(f"EM cascade K={K}: setup {t_setup:.1f}s (cascade transfers for {plan['nwin']} windows), "
+        f"device {t_run*1e3:.0f} ms -> {K/t_run:.1f} solves/s")
+    return {"K": K, "windows": plan["nwin"], "setup_host_s": round(t_setup, 1),
+            "ms_per_batch": round(t_run * 1e3, 1), "solves_per_s": round(K / t_run, 2),
+            "mean_photon_energy_density": e_ph, "cascade_iterations_max": int(max(g.em_iterations))}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -428,6 +461,11 @@ def main():
     if not args.no_uneven:
         uneven = uneven_leg(run, args.uneven_batch, torch, log)
 
+    # ---------------- EM cascade (shared schedule)
+    em = None
+    if not args.no_em:
+        em = em_leg(args.em_batch, torch, log)
+
     # ---------------- CPU baseline: oracle on a bounded sample of the batched workload
     cpu = None
     if not args.no_cpu:
@@ -488,6 +526,7 @@ def main():
         "adjoint_forward_ratio": adjoint["ratio"] if adjoint else None,
         "adjoint": adjoint,
         "uneven": uneven,
+        "em_cascade": em,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,

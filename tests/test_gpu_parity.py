@@ -217,3 +217,27 @@ def test_gpu_against_golden_fixture(name):
     if "photons" in g:
         dev, nb = rel_dev(s.state[run.em_slices()[22]], g["photons"])
         assert nb > 30 and dev <= TOL
+
+
+def test_ragged_batch_empty_column_partial_last_step():
+    """K = 13 (a ragged last group of right-hand sides), one column without any
+    source (must stay exactly zero), z 0.1234 -> 0.02 (the last step is short)."""
+    run = workloads.nuclear_run()
+    K = 13
+    srcs = workloads.batch_sources(run, K, seed=12)
+    g = MultiRHSPropagationSolverETD2(0.1234, 0.02, run, K=K)
+    r = OracleETD2(0.1234, 0.02, run, K=K)
+    for k in range(K):
+        if k == 6:
+            continue
+        g[k].add_source_class(srcs[k])
+        r.add_source_class(srcs[k], k)
+    g.solve(dz=1e-3)
+    r.solve(dz=1e-3)
+    assert abs(r.z_grid[-1] - 0.02) < 1e-15 and abs((r.z_grid[-2] - r.z_grid[-1]) - 1e-3) > 1e-6
+    assert not np.any(g.state[:, 6])
+    for k in range(K):
+        if k == 6:
+            continue
+        dev, nb = rel_dev(g.state[:, k], r.state[:, k])
+        assert nb > 100 and dev <= TOL, (k, dev)
