@@ -316,7 +316,7 @@ def em_leg(K, torch, log, seed=0):
     g.execute(plan)
     torch.cuda.synchronize()
     t_run = time.perf_counter() - t0
-    gam = plan["em_bufs"]["gam"].cpu().numpy()
+    gam = g.em_host_state(plan)[2 * run.em_grid.d:]
     E = run.em_grid.grid
     e_ph = float(np.mean((E * np.gradient(E))[:, None] * gam, axis=0).sum())
     log(f"EM cascade K={K}: setup {t_setup:.1f}s (cascade transfers for {plan['nwin']} windows), "

@@ -133,8 +133,10 @@ typedef struct prh_em {
   const double* rbh;    /* [nwin][d_em][d] Bethe-Heitler e+- shape (one sign) */
   const double* tg;     /* [nwin][d_em][d_em] photon transfer T_gamma */
   const double* te;     /* [nwin][d_em][d_em] e+- transfer T_electron */
-  double* gam;          /* [d_em][K] photon state, updated in place */
+  double* gam;          /* [d_em][K] photon state read by even steps, written by odd ones */
   double* wbh;          /* [d][K] scratch: Bethe-Heitler weights of the step */
+  double* gam2;         /* [d_em][K] photon state written by even steps, read by odd ones
+                         * (after step s the state is in gam2 if s is even, else gam) */
 } prh_em;
 
 /* Opaque handle of a captured step graph (prh_graph_create, prh_uneven_graph_create). */

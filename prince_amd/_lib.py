@@ -127,6 +127,7 @@ class PrhEm(ctypes.Structure):
         ("te", _p),
         ("gam", _p),
         ("wbh", _p),
+        ("gam2", _p),
     ]
 
 

@@ -1025,6 +1025,7 @@ void launch_wide(const prh_operator& op, const prh_schedule& sch, const prh_stat
 constexpr int EM_WAVES = 16;
 constexpr int EM_THREADS = EM_WAVES * WAVE;
 constexpr int EM_NB = (PRH_EM_MAX + EM_WAVES - 1) / EM_WAVES;
+constexpr int EM_RC = 4;  // workgroups (row chunks of the transfer) per 64-rhs group
 
 __global__ void __launch_bounds__(256) em_bh_weights(prh_operator op, prh_schedule sch,
                                                      prh_state st, prh_em em, int64_t sS,
@@ -1076,14 +1077,24 @@ __global__ void __launch_bounds__(EM_THREADS) em_step_kernel(prh_operator op, pr
   const int K = op.K;
   const int d = op.d;
   const int de = em.d_em;
-  const int k = blockIdx.x * WAVE + lane;
+  // workgroup = (64-rhs group, chunk of output rows of the transfer); every
+  // chunk redoes the photon step and the pair spectrum for its group in LDS
+  const int grp = blockIdx.x / EM_RC;
+  const int chunk = blockIdx.x % EM_RC;
+  const int rows = (de + EM_RC - 1) / EM_RC;
+  const int r0 = chunk * rows;
+  const int r1 = min(de, r0 + rows);
+  const int k = grp * WAVE + lane;
   const bool act = k < K;
+  // the photon state alternates between gam (read on even steps) and gam2
+  const double* __restrict__ gin = (step & 1) ? em.gam2 : em.gam;
+  double* __restrict__ gout = (step & 1) ? em.gam : em.gam2;
   const int w = sch.win[step];
   const double hh = sch.h[step];
   const double dl = sch.dldz[step];
   const double had = sch.hadia[step];
   const bool pdiff = (op.flags & PRH_F_PARTIAL_DIFF) != 0;
-  for (int i = wv; i < de; i += EM_WAVES) g0[i * WAVE + lane] = act ? em.gam[(int64_t)i * K + k] : 0.0;
+  for (int i = wv; i < de; i += EM_WAVES) g0[i * WAVE + lane] = act ? gin[(int64_t)i * K + k] : 0.0;
   __syncthreads();
 
   // F(y)_i = dl * sum_{o != 0} D[i][i+o] kappa_{i+o} y_{i+o},  kappa = H cm2sec E
@@ -1158,19 +1169,20 @@ __global__ void __launch_bounds__(EM_THREADS) em_step_kernel(prh_operator op, pr
     ga[i * WAVE + lane] = es[t] + es[t];
   }
   __syncthreads();
-  // cascade transfer: gam <- T_gamma gam + T_electron lep
+  // cascade transfer of this chunk's rows: gam <- T_gamma gam + T_electron lep
   const double* Tg = em.tg + (int64_t)w * de * de;
   const double* Te = em.te + (int64_t)w * de * de;
-  double ag[EM_NB], ae[EM_NB];
+  constexpr int NR = (PRH_EM_MAX / EM_RC + EM_WAVES) / EM_WAVES;
+  double ag[NR], ae[NR];
 #pragma unroll
-  for (int t = 0; t < EM_NB; ++t) ag[t] = ae[t] = 0.0;
+  for (int t = 0; t < NR; ++t) ag[t] = ae[t] = 0.0;
   for (int m = 0; m < de; ++m) {
     const double gm = g0[m * WAVE + lane];
     const double lm = ga[m * WAVE + lane];
 #pragma unroll
-    for (int t = 0; t < EM_NB; ++t) {
-      const int i = wv + EM_WAVES * t;
-      if (i < de) {
+    for (int t = 0; t < NR; ++t) {
+      const int i = r0 + wv + EM_WAVES * t;
+      if (i < r1) {
         ag[t] += Tg[(int64_t)i * de + m] * gm;
         ae[t] += Te[(int64_t)i * de + m] * lm;
       }
@@ -1178,9 +1190,9 @@ __global__ void __launch_bounds__(EM_THREADS) em_step_kernel(prh_operator op, pr
   }
   if (!act) return;
 #pragma unroll
-  for (int t = 0; t < EM_NB; ++t) {
-    const int i = wv + EM_WAVES * t;
-    if (i < de) em.gam[(int64_t)i * K + k] = ag[t] + ae[t];
+  for (int t = 0; t < NR; ++t) {
+    const int i = r0 + wv + EM_WAVES * t;
+    if (i < r1) gout[(int64_t)i * K + k] = ag[t] + ae[t];
   }
 }
 
@@ -1202,7 +1214,7 @@ void launch_em(const prh_operator& op, const prh_schedule& sch, const prh_state&
   const int nitems = op.d * ng;
   hipLaunchKernelGGL(em_bh_weights, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, s, op, sch,
                      st, em, sS, sJ, sK, step_local, use_ctl);
-  hipLaunchKernelGGL(em_step_kernel, dim3((unsigned)ng), dim3(EM_THREADS), 0, s, op, sch, st, em,
+  hipLaunchKernelGGL(em_step_kernel, dim3((unsigned)(ng * EM_RC)), dim3(EM_THREADS), 0, s, op, sch, st, em,
                      step_local, use_ctl);
 }
 
@@ -1230,7 +1242,7 @@ int32_t validate_em(const prh_operator* op, const prh_state* st, const char* who
   if (em->d_em < 1 || em->d_em > PRH_EM_MAX)
     return set_error(std::string(who) + ": d_em must be in [1, 144]");
   if (!em->egrid || !em->dcoef || !em->ddiag || !em->cbh || !em->bhw || !em->rbh || !em->tg ||
-      !em->te || !em->gam || !em->wbh)
+      !em->te || !em->gam || !em->gam2 || !em->wbh)
     return set_error(std::string(who) + ": EM sector with a null buffer");
   if (em->nwin < 1) return set_error(std::string(who) + ": EM sector without windows");
   (void)op;

@@ -348,7 +348,11 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
     def em_host_state(self, plan=None):
         """EM blocks (e+, e-, photon) of the last solve as a host array [3 d_em, K]."""
         plan = plan if plan is not None else self._plan
-        gam = plan["em_bufs"]["gam"].cpu().numpy()
+        # the photon state alternates between two buffers: after step s it is in
+        # gam2 when s is even
+        last = plan["nsteps"] - 1
+        key = "gam2" if last % 2 == 0 else "gam"
+        gam = plan["em_bufs"][key].cpu().numpy()
         z = np.zeros_like(gam)
         return np.concatenate([z, z, gam], axis=0)
 
@@ -730,13 +734,14 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             "bhw": dv(bhw, np.float64), "rbh": dv(Rbh, np.float64),
             "tg": dv(Tg, np.float64), "te": dv(Te, np.float64),
             "gam": dv(gam0, np.float64), "gam0": dv(gam0, np.float64),
+            "gam2": torch.zeros((eg.d, K), dtype=torch.float64, device=dev),
             "wbh": torch.zeros((run.cr_grid.d, K), dtype=torch.float64, device=dev),
         }
         em = _lib.PrhEm(
             d_em=eg.d, nwin=len(zs), egrid=b["egrid"].data_ptr(), dcoef=b["dcoef"].data_ptr(),
             ddiag=b["ddiag"].data_ptr(), cbh=b["cbh"].data_ptr(), bhw=b["bhw"].data_ptr(),
             rbh=b["rbh"].data_ptr(), tg=b["tg"].data_ptr(), te=b["te"].data_ptr(),
-            gam=b["gam"].data_ptr(), wbh=b["wbh"].data_ptr(),
+            gam=b["gam"].data_ptr(), wbh=b["wbh"].data_ptr(), gam2=b["gam2"].data_ptr(),
         )
         self.em_iterations = list(getattr(ck, "iterations", []))
         return em, b

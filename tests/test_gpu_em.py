@@ -25,13 +25,14 @@ def _split(run, state):
     return state[:n], state[sl[22]], state[sl[11]], state[sl[-11]]
 
 
-def test_em_cascade_proton_single():
+@pytest.mark.parametrize("zi", [0.05, 0.051])  # even and odd step counts (photon buffers alternate)
+def test_em_cascade_proton_single(zi):
     run = workloads.proton_em_run()
     src = workloads.proton_source(run)
-    g = UHECRPropagationSolverETD2(0.05, 0.0, run)
+    g = UHECRPropagationSolverETD2(zi, 0.0, run)
     g.add_source_class(src)
     g.solve(dz=1e-3)
-    o = OracleETD2(0.05, 0.0, run)
+    o = OracleETD2(zi, 0.0, run)
     o.add_source_class(src)
     o.solve(1e-3)
     nuc, gam, em_, ep_ = _split(run, g.state)
