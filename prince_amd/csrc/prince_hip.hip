@@ -1025,42 +1025,45 @@ void launch_wide(const prh_operator& op, const prh_schedule& sch, const prh_stat
 constexpr int EM_WAVES = 16;
 constexpr int EM_THREADS = EM_WAVES * WAVE;
 constexpr int EM_NB = (PRH_EM_MAX + EM_WAVES - 1) / EM_WAVES;
-constexpr int EM_RC = 4;  // workgroups (row chunks of the transfer) per 64-rhs group
+constexpr int EM_RC = 17;  // workgroups (row chunks of the transfer) per 64-rhs group
 
 __global__ void __launch_bounds__(256) em_bh_weights(prh_operator op, prh_schedule sch,
                                                      prh_state st, prh_em em, int64_t sS,
                                                      int64_t sJ, int64_t sK, int step_local,
                                                      int use_ctl) {
+  // workgroup = (bin j, 64-rhs group); the 4 waves sum interleaved quarters of
+  // the species (8 loads in flight each) and combine them in wave order
+  __shared__ double part[4][WAVE];
   int step;
   bool live;
   wide_step(st, step_local, use_ctl, step, live);
   if (!live) return;
   const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = threadIdx.x / WAVE;
   const int K = op.K;
   const int d = op.d;
   const int ng = (K + WAVE - 1) / WAVE;
-  const int item = blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE;
-  if (item >= d * ng) return;
-  const int j = item / ng;
-  const int k = (item % ng) * WAVE + lane;
-  if (k >= K) return;
-  const double* xp = st.x + j * sJ + k * sK;
-  double acc = 0.0;
-  int s = 0;
+  const int j = blockIdx.x / ng;
+  const int k = (blockIdx.x % ng) * WAVE + lane;
+  const bool act = k < K;
+  const double* xp = st.x + j * sJ + (act ? k : 0) * sK;
   const int ns = op.nspec;
-  for (; s + 4 <= ns; s += 4) {
-    const double x0 = xp[(int64_t)s * sS];
-    const double x1 = xp[(int64_t)(s + 1) * sS];
-    const double x2 = xp[(int64_t)(s + 2) * sS];
-    const double x3 = xp[(int64_t)(s + 3) * sS];
-    acc += em.cbh[s] * x0;
-    acc += em.cbh[s + 1] * x1;
-    acc += em.cbh[s + 2] * x2;
-    acc += em.cbh[s + 3] * x3;
+  double acc = 0.0;
+  int s = wv;
+  for (; s + 28 < ns; s += 32) {
+    double xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = act ? xp[(int64_t)(s + 4 * u) * sS] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += em.cbh[s + 4 * u] * xv[u];
   }
-  for (; s < ns; ++s) acc += em.cbh[s] * xp[(int64_t)s * sS];
+  for (; s < ns; s += 4) acc += em.cbh[s] * (act ? xp[(int64_t)s * sS] : 0.0);
+  part[wv][lane] = acc;
+  __syncthreads();
+  if (wv != 0 || !act) return;
+  const double tot = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
   const int w = sch.win[step];
-  em.wbh[(int64_t)j * K + k] = em.bhw[(int64_t)w * d + j] * acc;
+  em.wbh[(int64_t)j * K + k] = em.bhw[(int64_t)w * d + j] * tot;
 }
 
 __global__ void __launch_bounds__(EM_THREADS) em_step_kernel(prh_operator op, prh_schedule sch,
@@ -1211,9 +1214,8 @@ void launch_em(const prh_operator& op, const prh_schedule& sch, const prh_state&
     sK = d;
   }
   const int ng = (op.K + WAVE - 1) / WAVE;
-  const int nitems = op.d * ng;
-  hipLaunchKernelGGL(em_bh_weights, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, s, op, sch,
-                     st, em, sS, sJ, sK, step_local, use_ctl);
+  hipLaunchKernelGGL(em_bh_weights, dim3((unsigned)(op.d * ng)), dim3(256), 0, s, op, sch, st, em,
+                     sS, sJ, sK, step_local, use_ctl);
   hipLaunchKernelGGL(em_step_kernel, dim3((unsigned)(ng * EM_RC)), dim3(EM_THREADS), 0, s, op, sch, st, em,
                      step_local, use_ctl);
 }
