@@ -246,7 +246,8 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
     def wg_waves_for(self, K, with_adjoint=False):
         """Waves per stage workgroup: 8 from K = 64 on (more workgroups per CU:
         151 vs 165 us per stage at K = 512), 16 below (rows split into up to 16
-        parts: 34 vs 43 us at K = 1) and for adjoint plans."""
+        parts: 34 vs 43 us at K = 1) and for adjoint plans (two-field forward at
+        K = 64: 845 vs 1028 ms per checkpointed solve)."""
         tuned = getattr(self, "_tuned", {}).get(("wg_waves", K))
         if tuned is not None:
             return tuned
