@@ -465,6 +465,7 @@ __global__ void __launch_bounds__(WGW * WAVE) etd2_stage(prh_operator op, prh_sc
         const int k = k0 + kk;
         if (k >= K || !active) continue;
         double sum = 0.0;
+#pragma unroll 8
         for (int c = 0; c < op.ndchunk; ++c)
           sum += st.dpart[(((int64_t)c * op.ndense + r) * K + k) * d + lane];
         acc[kk] = acc[kk] + sum;

@@ -241,24 +241,28 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         tuned = getattr(self, "_tuned", {}).get(("part_cost", K))
         if tuned is not None:
             return tuned
-        return 24 if K == 1 else 96
+        return 12 if K < 8 else 96
 
     def wg_waves_for(self, K, with_adjoint=False):
         """Waves per stage workgroup: 8 from K = 64 on (more workgroups per CU:
-        151 vs 165 us per stage at K = 512), 16 below (rows split into up to 16
-        parts: 34 vs 43 us at K = 1) and for adjoint plans (two-field forward at
-        K = 64: 845 vs 1028 ms per checkpointed solve)."""
+        151 vs 165 us per stage at K = 512) and below K = 8 together with the
+        dense-row pass (plain nuclear solve 115 ms); 16 in between and for
+        adjoint plans (two-field forward at K = 64: 845 vs 1028 ms per
+        checkpointed solve)."""
         tuned = getattr(self, "_tuned", {}).get(("wg_waves", K))
         if tuned is not None:
             return tuned
-        return 8 if (K >= 64 and not with_adjoint) else 16
+        if with_adjoint:
+            return 16
+        return 8 if (K >= 64 or K < 8) else 16
 
     def dense_rows_for(self, K):
-        """Use the mother-major pass for the dense rows (off by default: at K = 512
-        it measured 183 us per stage against 152 us with the dense rows cut into
-        ordinary parts, the pass being bound by its serial mother chain)."""
+        """Use the mother-major pass for the dense rows: on for K < 8, where the
+        dense rows set the stage's critical path (plain nuclear solve 190 ->
+        115 ms with 4-mother chunks); off from K = 8 on, where the pass's extra
+        traffic costs more than it saves (167 vs 152 us per stage at K = 512)."""
         tuned = getattr(self, "_tuned", {}).get(("dense", K))
-        return bool(tuned) if tuned is not None else False
+        return bool(tuned) if tuned is not None else K < 8
 
     def kt_for(self, K):
         """Right-hand sides per wave: the largest of 8, 4, 2, 1 not above K."""
@@ -608,7 +612,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             "fphi": torch.zeros(sshape, dtype=torch.float64, device=dev),
             "step_ctl": torch.zeros(2, dtype=torch.int32, device=dev),
         }
-        DENSE_CHUNK = getattr(self, "_tuned", {}).get(("dense_chunk", K), 16)
+        DENSE_CHUNK = getattr(self, "_tuned", {}).get(("dense_chunk", K), 4 if K < 8 else 16)
         ndchunk = -(-nspec // DENSE_CHUNK)
         if dense is not None:
             bufs["dense_row"] = dv(dense[1], np.int32)
@@ -792,10 +796,11 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         kts = [kt for kt in (8, 4, 2, 1) if kt <= K][:2]
         out = []
         for kt in kts:
-            for pc in (48, 96) if K > 1 else (12, 24):
-                for wgw in ((8, 16) if K >= 8 else (16,)):
-                    out.append({"layout": "narrow", ("kt", K): kt, ("part_cost", K): pc,
-                                ("wg_waves", K): wgw})
+            for pc in (48, 96) if K >= 8 else (12, 24):
+                for wgw in (8, 16):
+                    for dense in ((False,) if K >= 8 else (True, False)):
+                        out.append({"layout": "narrow", ("kt", K): kt, ("part_cost", K): pc,
+                                    ("wg_waves", K): wgw, ("dense", K): dense})
         saved = getattr(self, "_tuned", None)
         self._tuned = {"layout": "wide"}
         try:
