@@ -66,8 +66,9 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
     given).
     """
 
-    #: right-hand sides per wave of the uneven stage kernel
-    kt_uneven = 4
+    #: right-hand sides per wave of the uneven stage kernel (2: 2.18 vs 2.26 s
+    #: for the first 1024 steps of K = 1024 columns with 4)
+    kt_uneven = 2
 
     def __new__(cls, *args, **kwargs):
         return object.__new__(cls)
