@@ -65,20 +65,26 @@ def parse():
     return p.parse_args()
 
 
-def measured_traffic(path, kernel_names):
+def measured_traffic(path, stage_groups):
     """HBM bytes per stage launch from a committed rocprofv3 counter summary:
     FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each divided by what the same
-    counter read per byte for an elementwise copy of known size, averaged over
-    the stage kernels."""
+    counter read per byte for an elementwise copy of known size, summed over
+    the kernels of one stage and averaged over the two stages."""
     if not path or not os.path.exists(path):
         return None
     with open(path) as f:
         rec = json.load(f)
-    ks = [v for k, v in rec["kernels"].items() if any(n in k for n in kernel_names)]
-    if not ks:
-        return None
-    per = [1024.0 * (v["FETCH_SIZE_KiB"] / rec["fetch_factor"] + v["WRITE_SIZE_KiB"] / rec["write_factor"])
-           for v in ks]
+    per = []
+    for names in stage_groups:
+        tot = 0.0
+        for n in names:
+            ks = [v for k, v in rec["kernels"].items() if n in k]
+            if len(ks) != 1:
+                return None
+            v = ks[0]
+            tot += 1024.0 * (v["FETCH_SIZE_KiB"] / rec["fetch_factor"]
+                             + v["WRITE_SIZE_KiB"] / rec["write_factor"])
+        per.append(tot)
     return float(np.mean(per))
 
 
@@ -134,12 +140,22 @@ def _cfg_str(cfg):
 
 
 def stage_kernel_names(plan):
-    """Names (as rocprofv3 prints them) of the two stage kernels a plan launches."""
+    """Per stage (1, 2): the names (as rocprofv3 prints them) of the kernels
+    one stage launch of the plan runs."""
     if plan.get("wide"):
-        return ["etd2_wide<false>", "etd2_wide<true>"]
+        comb = plan.get("wheavy") is not None and len(plan["wheavy"]) > 0
+        return [[f"etd2_wide<{s2}>"] + ([f"etd2_wide_combine<{s2}>"] if comb else [])
+                for s2 in ("false", "true")]
     kt, two = plan["kt"], "true" if plan["two"] else "false"
     wgw = plan["op"].wg_waves or 16
-    return [f"etd2_stage<false, {kt}, {two}, {wgw}>", f"etd2_stage<true, {kt}, {two}, {wgw}>"]
+    dm = {None: 0, "pass": 1, "chunks": 2}[plan.get("dense_mode")]
+    out = []
+    for s2 in ("false", "true"):
+        names = [f"etd2_stage<{s2}, {kt}, {two}, {wgw}, {dm}>"]
+        if dm == 2:
+            names.append(f"dense_finish<{s2}, {kt}, {wgw}>")
+        out.append(names)
+    return out
 
 
 def fold_flops(plan):

@@ -98,7 +98,23 @@ typedef struct prh_operator {
   int32_t ndchunk;
   const int32_t* dense_row;
   const int32_t* dense_tab;
+  /* Dense rows cut into mother chunks inside the stage kernel (nfin > 0, in
+   * place of the mother-major pass; dense_tab and dchunk unused): a task
+   * whose kind carries PRH_TASK_TO_DPART sums the blocks [b_begin, b_end) of
+   * row da and stores them as chunk tasks[3] at dpart[chunk][dense_row[da]];
+   * a second launch over fin_tasks [nfin][6] (nfin a multiple of wg_waves)
+   * runs those rows' epilogues: each row is nparts waves with an empty block
+   * range, wave p adding chunks [p ndchunk / nparts, (p+1) ndchunk / nparts)
+   * in order before the parts are summed in part order. */
+  int32_t nfin;
+  int32_t pad3_;
+  const int32_t* fin_tasks;
 } prh_operator;
+
+/* Stage task = {da, b_begin, b_end, lead wave, nparts, kind}; kind's low
+ * bits: 0 diagonal blocks only, 1 blocks of <= 8 diagonals, 2 general. */
+#define PRH_TASK_KIND_MASK 3
+#define PRH_TASK_TO_DPART 4 /* dense-row chunk task (see nfin) */
 
 #define PRH_F_PARTIAL_DIFF 1 /* continuous-loss stencil term on */
 #define PRH_F_INJECTION 2    /* source term b(z) on */

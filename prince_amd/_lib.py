@@ -36,6 +36,8 @@ WG_WAVES = 16
 
 PRH_F_PARTIAL_DIFF = 1
 PRH_F_INJECTION = 2
+PRH_TASK_KIND_MASK = 3
+PRH_TASK_TO_DPART = 4
 
 _p = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -79,6 +81,9 @@ class PrhOperator(ctypes.Structure):
         ("ndchunk", _i32),
         ("dense_row", _p),
         ("dense_tab", _p),
+        ("nfin", _i32),
+        ("pad3_", _i32),
+        ("fin_tasks", _p),
     ]
 
 

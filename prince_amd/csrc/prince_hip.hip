@@ -128,6 +128,23 @@ __device__ __forceinline__ double shfl_or_zero(double v, int src, int d) {
   return (src >= 0 && src < d) ? r : 0.0;
 }
 
+// Whole-wave lane shifts by one (DPP wave_shl:1 / wave_shr:1, VALU moves, no
+// LDS): lane i receives lane i+1 (lane_next; lane 63 receives 0) or lane i-1
+// (lane_prev; lane 0 receives 0).  Callers keep lanes >= d at zero, so a chain
+// of shifts equals shfl_or_zero at offsets of one sign.  All 64 lanes must be
+// active.
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
+template <int CTRL>
+__device__ __forceinline__ double lane_shift1(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_next(double v) { return lane_shift1<DPP_WAVE_SHL1>(v); }
+__device__ __forceinline__ double lane_prev(double v) { return lane_shift1<DPP_WAVE_SHR1>(v); }
+
 // Blocks whose loads are issued together before any of them is used: runs
 // of diagonal blocks and of band blocks with at most NSMAX diagonals.  With
 // two photon-field components every block has two value sets.
@@ -241,15 +258,38 @@ __device__ __forceinline__ void couple_part(const prh_operator& op, const double
       }
 #pragma unroll
       for (int u = 0; u < UB; ++u) {
+        if (olo[u] >= 0) {
+          // diagonal s2 reads lane + olo + s2: one shuffle to the first
+          // diagonal (none when olo = 0), then one-lane DPP shifts
+          if (olo[u] != 0) {
 #pragma unroll
-        for (int s2 = 0; s2 < NSMAX; ++s2) {
-          if (s2 < ns[u]) {
-            const int src = lane + olo[u] + s2;
+            for (int kk = 0; kk < KT; ++kk) xv[u][kk] = shfl_or_zero(xv[u][kk], lane + olo[u], d);
+          }
 #pragma unroll
-            for (int kk = 0; kk < KT; ++kk) {
-              const double xs = shfl_or_zero(xv[u][kk], src, d);
-              accC[kk] += mc[u][s2] * xs;
-              if (TWO) accE[kk] += me[u][TWO ? s2 : 0] * xs;
+          for (int s2 = 0; s2 < NSMAX; ++s2) {
+            if (s2 < ns[u]) {
+#pragma unroll
+              for (int kk = 0; kk < KT; ++kk) {
+                accC[kk] += mc[u][s2] * xv[u][kk];
+                if (TWO) accE[kk] += me[u][TWO ? s2 : 0] * xv[u][kk];
+              }
+              if (s2 + 1 < ns[u]) {
+#pragma unroll
+                for (int kk = 0; kk < KT; ++kk) xv[u][kk] = lane_next(xv[u][kk]);
+              }
+            }
+          }
+        } else {
+#pragma unroll
+          for (int s2 = 0; s2 < NSMAX; ++s2) {
+            if (s2 < ns[u]) {
+              const int src = lane + olo[u] + s2;
+#pragma unroll
+              for (int kk = 0; kk < KT; ++kk) {
+                const double xs = shfl_or_zero(xv[u][kk], src, d);
+                accC[kk] += mc[u][s2] * xs;
+                if (TWO) accE[kk] += me[u][TWO ? s2 : 0] * xs;
+              }
             }
           }
         }
@@ -407,12 +447,19 @@ __device__ __forceinline__ RowCoef row_coef_from(const RowIn& ri, double s, bool
   return r;
 }
 
-// step_local: step index (use_ctl = 0) or offset from step_ctl[0] (use_ctl = 1,
-// graph replay; steps at or beyond step_ctl[1] do nothing).
-template <bool STAGE2, int KT, bool TWO, int WGW>
-__global__ void __launch_bounds__(WGW * WAVE) etd2_stage(prh_operator op, prh_schedule sch,
-                                                         prh_state st, int step_local,
-                                                         int use_ctl) {
+// One stage for the tasks `tasks` (op.tasks, or op.fin_tasks for the dense
+// rows' epilogues).  step_local: step index (use_ctl = 0) or offset from
+// step_ctl[0] (use_ctl = 1, graph replay; steps at or beyond step_ctl[1] do
+// nothing).
+// Dense-row handling compiled into a stage kernel: none, the mother-major
+// pass's chunk sums in the epilogue, chunk tasks storing to dpart, or the
+// dense rows' epilogue launch.
+enum DenseMode { DM_NONE = 0, DM_PASS = 1, DM_CHUNKS = 2, DM_FINISH = 3 };
+
+template <bool STAGE2, int KT, bool TWO, int WGW, int DM>
+__device__ __forceinline__ void stage_body(const prh_operator& op, const prh_schedule& sch,
+                                           const prh_state& st, const int* __restrict__ tasks,
+                                           int ntasks, int step_local, int use_ctl) {
   __shared__ double red[WGW][KT][WAVE];
   int step = step_local;
   int ridx = -1;
@@ -429,9 +476,9 @@ __global__ void __launch_bounds__(WGW * WAVE) etd2_stage(prh_operator op, prh_sc
   const int K = op.K;
   const int d = op.d;
   int gx, gy;
-  wg_coords(op.ntasks, K, KT, gx, gy, WGW);
+  wg_coords(ntasks, K, KT, gx, gy, WGW);
   const int k0 = gy * KT;
-  const int* tk = op.tasks + 6 * ((int64_t)gx * WGW + wv);
+  const int* tk = tasks + 6 * ((int64_t)gx * WGW + wv);
   const int da = tk[0];
   const bool active = lane < d;
 
@@ -442,7 +489,8 @@ __global__ void __launch_bounds__(WGW * WAVE) etd2_stage(prh_operator op, prh_sc
 
   double acc[KT], accE[KT];
   if (da >= 0) {
-    couple_part<KT, TWO>(op, xin, mvC, mvE, tk[1], tk[2], tk[5], k0, lane, active, acc, accE);
+    couple_part<KT, TWO>(op, xin, mvC, mvE, tk[1], tk[2], tk[5] & PRH_TASK_KIND_MASK, k0, lane,
+                         active, acc, accE);
   } else {
 #pragma unroll
     for (int kk = 0; kk < KT; ++kk) acc[kk] = accE[kk] = 0.0;
@@ -454,9 +502,36 @@ __global__ void __launch_bounds__(WGW * WAVE) etd2_stage(prh_operator op, prh_sc
     sc_[kk] = TWO ? op.escale[k] : 0.0;
     if (TWO) acc[kk] = acc[kk] + sc_[kk] * accE[kk];
   }
+  if (DM == DM_FINISH && da >= 0) {
+    // dense row: this part's share of the mother-chunk partial sums, in chunk
+    // order (the parts' shares are then added in part order)
+    const int r = op.dense_row[da];
+    const int np = tk[4];
+    const int pi = wv - tk[3];
+    const int c1 = (int)((int64_t)(pi + 1) * op.ndchunk / np);
+#pragma unroll 4
+    for (int c = (int)((int64_t)pi * op.ndchunk / np); c < c1; ++c) {
+      const double* dp = st.dpart + ((int64_t)c * op.ndense + r) * K * d;
+#pragma unroll
+      for (int kk = 0; kk < KT; ++kk) {
+        const int k = k0 + kk;
+        acc[kk] += (k < K && active) ? dp[(int64_t)k * d + lane] : 0.0;
+      }
+    }
+  }
   reduce_parts<KT>(red, wv, lane, tk[3], tk[4], acc);
+  if (DM == DM_CHUNKS && da >= 0 && (tk[5] & PRH_TASK_TO_DPART)) {
+    // one mother chunk of a dense row: store its partial sum (chunk tk[3])
+    double* dp = st.dpart + ((int64_t)tk[3] * op.ndense + op.dense_row[da]) * K * d;
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) {
+      const int k = k0 + kk;
+      if (k < K && active) dp[(int64_t)k * d + lane] = acc[kk];
+    }
+    return;
+  }
   if (da < 0 || wv != tk[3]) return;
-  if (!TWO && op.ndense > 0) {
+  if (DM == DM_PASS) {
     // dense rows: add the mother-chunk partial sums of dense_pass in chunk order
     const int r = op.dense_row[da];
     if (r >= 0) {
@@ -505,11 +580,19 @@ __global__ void __launch_bounds__(WGW * WAVE) etd2_stage(prh_operator op, prh_sc
     double out = acc[kk];
     if (pdiff) {
       const double kx = rc.kap * xown;
+      // stencil neighbours kx[lane + o], o = -3..3 (zero outside [0, d)),
+      // added in the order o = -3, -2, -1, 1, 2, 3
+      const double km1 = lane_prev(kx);
+      const double km2 = lane_prev(km1);
+      const double km3 = lane_prev(km2);
       double dx = 0.0;
+      if (dcf[0] != 0.0) dx += dcf[0] * km3;
+      if (dcf[1] != 0.0) dx += dcf[1] * km2;
+      if (dcf[2] != 0.0) dx += dcf[2] * km1;
+      double kxo = kx;
 #pragma unroll
-      for (int o = -3; o <= 3; ++o) {
-        if (o == 0) continue;
-        const double kxo = shfl_or_zero(kx, lane + o, d);
+      for (int o = 1; o <= 3; ++o) {
+        kxo = lane_next(kxo);
         if (dcf[o + 3] != 0.0) dx += dcf[o + 3] * kxo;
       }
       out = out + dx;
@@ -549,6 +632,30 @@ __global__ void __launch_bounds__(WGW * WAVE) etd2_stage(prh_operator op, prh_sc
       if (record) st.rec_fa[ridx * nk + own] = out;
     }
   }
+}
+
+// Minimum waves per SIMD asked of the compiler: 6 for the one-field,
+// 8-rhs, no-dense-row stage (80 VGPRs instead of 87 with the band DPP chain:
+// 120 / 140 vs 137 / 147 us per stage at K = 512), otherwise its own choice.
+constexpr int stage_min_waves(int kt, bool two, int dm) {
+  return (kt == 8 && !two && dm == DM_NONE) ? 6 : 1;
+}
+
+template <bool STAGE2, int KT, bool TWO, int WGW, int DM>
+__global__ void __launch_bounds__(WGW * WAVE)
+    __attribute__((amdgpu_waves_per_eu(stage_min_waves(KT, TWO, DM))))
+    etd2_stage(prh_operator op, prh_schedule sch, prh_state st, int step_local, int use_ctl) {
+  stage_body<STAGE2, KT, TWO, WGW, DM>(op, sch, st, op.tasks, op.ntasks, step_local, use_ctl);
+}
+
+// Epilogues of the dense rows whose mother chunks the stage kernel stored in
+// dpart: the row's nparts waves each add a share of the chunks, in order.
+template <bool STAGE2, int KT, int WGW>
+__global__ void __launch_bounds__(WGW * WAVE) dense_finish(prh_operator op, prh_schedule sch,
+                                                           prh_state st, int step_local,
+                                                           int use_ctl) {
+  stage_body<STAGE2, KT, false, WGW, DM_FINISH>(op, sch, st, op.fin_tasks, op.nfin, step_local,
+                                                use_ctl);
 }
 
 // ---------------------------------------------------------------------------
@@ -1424,11 +1531,19 @@ __global__ void __launch_bounds__(STHREADS) etd2_uneven(prh_operator op, prh_une
     double out = acc[kk];
     if (pdiff) {
       const double kx = kap * xown;
+      // stencil neighbours kx[lane + o], o = -3..3 (zero outside [0, d)),
+      // added in the order o = -3, -2, -1, 1, 2, 3
+      const double km1 = lane_prev(kx);
+      const double km2 = lane_prev(km1);
+      const double km3 = lane_prev(km2);
       double dx = 0.0;
+      if (dcf[0] != 0.0) dx += dcf[0] * km3;
+      if (dcf[1] != 0.0) dx += dcf[1] * km2;
+      if (dcf[2] != 0.0) dx += dcf[2] * km1;
+      double kxo = kx;
 #pragma unroll
-      for (int o = -3; o <= 3; ++o) {
-        if (o == 0) continue;
-        const double kxo = shfl_or_zero(kx, lane + o, d);
+      for (int o = 1; o <= 3; ++o) {
+        kxo = lane_next(kxo);
         if (dcf[o + 3] != 0.0) dx += dcf[o + 3] * kxo;
       }
       out = out + dx;
@@ -1602,13 +1717,22 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
     if (op->nsrc > 4) return set_error(std::string(who) + ": the wide layout takes at most 4 sources per rhs");
     if (st->rec_x) return set_error(std::string(who) + ": the wide layout does not record steps");
   }
+  if (op->nfin != 0) {
+    if (op->nfin < 0 || op->nfin % wg_waves_of(*op) != 0 || !op->fin_tasks)
+      return set_error(std::string(who) + ": nfin must be a multiple of the workgroup waves, with fin_tasks");
+    if (op->ndense < 1)
+      return set_error(std::string(who) + ": dense-row chunk tasks need ndense >= 1");
+  }
   if (op->ndense > 0) {
-    if (op->ndense > DENSE_MAX) return set_error(std::string(who) + ": at most 2 dense rows");
+    if (op->nfin == 0 && op->ndense > DENSE_MAX)
+      return set_error(std::string(who) + ": at most 2 dense rows for the mother-major pass");
     if (op->wide || op->mvals_e || op->kt > 8)
       return set_error(std::string(who) + ": dense rows are for the narrow one-field stages (kt <= 8)");
-    if (!op->dense_row || !op->dense_tab || !st->dpart || op->dchunk < 1 || op->ndchunk < 1 ||
-        (int64_t)op->dchunk * op->ndchunk < op->nspec)
-      return set_error(std::string(who) + ": dense rows need dense_row, dense_tab, dpart and chunks covering the species");
+    if (!op->dense_row || !st->dpart || op->ndchunk < 1)
+      return set_error(std::string(who) + ": dense rows need dense_row, dpart and ndchunk >= 1");
+    if (op->nfin == 0 && (!op->dense_tab || op->dchunk < 1 ||
+                          (int64_t)op->dchunk * op->ndchunk < op->nspec))
+      return set_error(std::string(who) + ": the mother-major pass needs dense_tab and chunks covering the species");
     if (std::string(who).rfind("prh_adjoint", 0) == 0)
       return set_error(std::string(who) + ": the adjoint operator has no dense rows");
   }
@@ -1618,33 +1742,74 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
   return validate_em(op, st, who);
 }
 
-template <bool S2, bool TWO, int WGW>
-void launch_stage_wg(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
+template <bool S2, bool TWO, int WGW, int DM>
+void launch_stage_dm(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
                      int step_local, int use_ctl, hipStream_t s) {
   const int64_t nwg = (int64_t)(op.ntasks / WGW) * ((op.K + op.kt - 1) / op.kt);
   const dim3 grid((unsigned)nwg);
   constexpr int STH = WGW * WAVE;
   switch (op.kt) {
     case 1:
-      hipLaunchKernelGGL((etd2_stage<S2, 1, TWO, WGW>), grid, dim3(STH), 0, s, op, sch, st,
+      hipLaunchKernelGGL((etd2_stage<S2, 1, TWO, WGW, DM>), grid, dim3(STH), 0, s, op, sch, st,
                          step_local, use_ctl);
       break;
     case 2:
-      hipLaunchKernelGGL((etd2_stage<S2, 2, TWO, WGW>), grid, dim3(STH), 0, s, op, sch, st,
+      hipLaunchKernelGGL((etd2_stage<S2, 2, TWO, WGW, DM>), grid, dim3(STH), 0, s, op, sch, st,
                          step_local, use_ctl);
       break;
     case 4:
-      hipLaunchKernelGGL((etd2_stage<S2, 4, TWO, WGW>), grid, dim3(STH), 0, s, op, sch, st,
+      hipLaunchKernelGGL((etd2_stage<S2, 4, TWO, WGW, DM>), grid, dim3(STH), 0, s, op, sch, st,
                          step_local, use_ctl);
       break;
     case 16:
-      if constexpr (!TWO) {
-        hipLaunchKernelGGL((etd2_stage<S2, 16, false, WGW>), grid, dim3(STH), 0, s, op, sch, st,
-                           step_local, use_ctl);
+      if constexpr (!TWO && DM == DM_NONE) {
+        hipLaunchKernelGGL((etd2_stage<S2, 16, false, WGW, DM_NONE>), grid, dim3(STH), 0, s, op,
+                           sch, st, step_local, use_ctl);
       }
       break;
     default:
-      hipLaunchKernelGGL((etd2_stage<S2, 8, TWO, WGW>), grid, dim3(STH), 0, s, op, sch, st,
+      hipLaunchKernelGGL((etd2_stage<S2, 8, TWO, WGW, DM>), grid, dim3(STH), 0, s, op, sch, st,
+                         step_local, use_ctl);
+      break;
+  }
+}
+
+// dense-row mode of a validated operator (dense rows are one-field, kt <= 8)
+template <bool S2, bool TWO, int WGW>
+void launch_stage_wg(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
+                     int step_local, int use_ctl, hipStream_t s) {
+  if constexpr (!TWO) {
+    if (op.ndense > 0) {
+      if (op.nfin > 0)
+        launch_stage_dm<S2, false, WGW, DM_CHUNKS>(op, sch, st, step_local, use_ctl, s);
+      else
+        launch_stage_dm<S2, false, WGW, DM_PASS>(op, sch, st, step_local, use_ctl, s);
+      return;
+    }
+  }
+  launch_stage_dm<S2, TWO, WGW, DM_NONE>(op, sch, st, step_local, use_ctl, s);
+}
+
+template <bool S2, int WGW>
+void launch_finish_wg(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
+                      int step_local, int use_ctl, hipStream_t s) {
+  const dim3 grid((unsigned)((int64_t)(op.nfin / WGW) * ((op.K + op.kt - 1) / op.kt)));
+  constexpr int STH = WGW * WAVE;
+  switch (op.kt) {
+    case 1:
+      hipLaunchKernelGGL((dense_finish<S2, 1, WGW>), grid, dim3(STH), 0, s, op, sch, st,
+                         step_local, use_ctl);
+      break;
+    case 2:
+      hipLaunchKernelGGL((dense_finish<S2, 2, WGW>), grid, dim3(STH), 0, s, op, sch, st,
+                         step_local, use_ctl);
+      break;
+    case 4:
+      hipLaunchKernelGGL((dense_finish<S2, 4, WGW>), grid, dim3(STH), 0, s, op, sch, st,
+                         step_local, use_ctl);
+      break;
+    default:
+      hipLaunchKernelGGL((dense_finish<S2, 8, WGW>), grid, dim3(STH), 0, s, op, sch, st,
                          step_local, use_ctl);
       break;
   }
@@ -1670,7 +1835,7 @@ void launch_stage_kt(const prh_operator& op, const prh_schedule& sch, const prh_
     launch_stage_two<S2, true>(op, sch, st, step_local, use_ctl, s);
     return;
   }
-  if (op.ndense > 0) {
+  if (op.ndense > 0 && op.nfin == 0) {
     switch (op.kt) {
       case 1: launch_dense<1>(op, sch, st, S2, step_local, use_ctl, s); break;
       case 2: launch_dense<2>(op, sch, st, S2, step_local, use_ctl, s); break;
@@ -1679,6 +1844,12 @@ void launch_stage_kt(const prh_operator& op, const prh_schedule& sch, const prh_
     }
   }
   launch_stage_two<S2, false>(op, sch, st, step_local, use_ctl, s);
+  if (op.nfin > 0) {
+    if (wg_waves_of(op) == 8)
+      launch_finish_wg<S2, 8>(op, sch, st, step_local, use_ctl, s);
+    else
+      launch_finish_wg<S2, 16>(op, sch, st, step_local, use_ctl, s);
+  }
 }
 
 template <bool S2, bool TWO>
@@ -1715,7 +1886,7 @@ struct prh_graph {
 
 extern "C" {
 
-int32_t prh_version(void) { return 4; }
+int32_t prh_version(void) { return 5; }
 
 const char* prh_last_error(void) { return g_last_error.c_str(); }
 

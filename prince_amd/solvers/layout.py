@@ -191,6 +191,96 @@ def build_tasks(layout: BlockLayout, part_cost=96, wg_waves=16):
     return np.asarray(tasks, dtype=np.int32).reshape(-1, 6)
 
 
+TASK_TO_DPART = 4  # kind flag of a dense-row chunk task
+
+
+def _pack_first_fit(groups, wg_waves):
+    """Pack task groups (each into one workgroup) first fit, in the given order."""
+    wgs = []
+    for grp in groups:
+        target = None
+        for g in wgs:
+            if wg_waves - len(g) >= len(grp):
+                target = g
+                break
+        if target is None:
+            target = []
+            wgs.append(target)
+        lead = len(target)
+        for t in grp:
+            t = list(t)
+            if not t[5] & TASK_TO_DPART:
+                t[3] = lead
+            target.append(t)
+    tasks = []
+    for g in wgs:
+        tasks.extend(g)
+        tasks.extend([[-1, 0, 0, 0, 0, 0]] * (wg_waves - len(g)))
+    return np.asarray(tasks, dtype=np.int32).reshape(-1, 6)
+
+
+def build_chunked_tasks(layout: BlockLayout, part_cost=96, wg_waves=16, min_blocks=100,
+                        max_rows=4, chunk_cost=96, place="mother"):
+    """Stage tasks with the dense rows cut into mother chunks.
+
+    Rows with at least ``min_blocks`` blocks (at most ``max_rows``) are cut
+    into ``nchunk`` contiguous block ranges of similar cost (about
+    ``chunk_cost`` each; every dense row gets the same number).  Each range is
+    a one-wave task flagged TASK_TO_DPART whose lead field holds the chunk
+    index.  ``place="mother"`` puts it among the ordinary rows next to row
+    ``first mother of the range`` (it reads mother panels that the
+    neighbouring rows pull into L2 as well); ``place="first"`` puts all chunk
+    tasks ahead of the ordinary rows.  The ordinary rows are cut and packed as
+    in ``build_tasks`` (split rows first fit, then single rows in species
+    order).
+
+    Returns None (no dense row) or ``(tasks, fin_tasks, rows, row_of,
+    nchunk)``: ``fin_tasks`` holds the dense rows' epilogues, ``min(wg_waves,
+    nchunk)`` waves each with an empty block range.
+    """
+    blocks, brow_ptr = layout.blocks, layout.brow_ptr
+    nspec = layout.nspec
+    counts = np.diff(np.asarray(brow_ptr, dtype=np.int64))
+    order = np.argsort(-counts, kind="stable")[:max_rows]
+    rows = sorted(int(r) for r in order if counts[r] >= min_blocks)
+    if not rows:
+        return None
+    cost = block_cost(blocks)
+    row_of = np.full(nspec, -1, dtype=np.int32)
+    nchunk = 1
+    for ri, da in enumerate(rows):
+        row_of[da] = ri
+        b0, b1 = int(brow_ptr[da]), int(brow_ptr[da + 1])
+        nchunk = max(nchunk, -(-int(cost[b0:b1].sum()) // chunk_cost))
+    nchunk = min(nchunk, int(counts[rows].min()))
+    chunks_at = {}
+    for da in rows:
+        b0, b1 = int(brow_ptr[da]), int(brow_ptr[da + 1])
+        for c, (c0, c1) in enumerate(split_row(cost, b0, b1, nchunk)):
+            kind = part_kind(blocks, c0, c1) | TASK_TO_DPART
+            at = 0 if place == "first" else int(blocks[c0, 0])
+            chunks_at.setdefault(at, []).append([da, c0, c1, c, 1, kind])
+    if place not in ("mother", "first"):
+        raise ValueError(f"unknown chunk placement {place!r}")
+    multi, singles = [], []
+    for da in range(nspec):
+        for t in chunks_at.get(da, []):
+            singles.append([t])
+        if row_of[da] >= 0:
+            continue
+        b0, b1 = int(brow_ptr[da]), int(brow_ptr[da + 1])
+        total = int(cost[b0:b1].sum()) if b1 > b0 else 0
+        nparts = max(1, min(wg_waves, -(-total // part_cost), b1 - b0))
+        parts = split_row(cost, b0, b1, nparts)
+        grp = [[da, p0, p1, 0, len(parts), part_kind(blocks, p0, p1)] for p0, p1 in parts]
+        (multi if len(parts) > 1 else singles).append(grp)
+    multi.sort(key=lambda g: -len(g))
+    tasks = _pack_first_fit(multi + singles, wg_waves)
+    nf = min(wg_waves, nchunk)
+    fin = _pack_first_fit([[[da, 0, 0, 0, nf, 0]] * nf for da in rows], wg_waves)
+    return tasks, fin, rows, row_of, nchunk
+
+
 def dense_split(blocks, brow_ptr, nspec, min_blocks=100, max_rows=2, max_slots=8):
     """Rows with at least ``min_blocks`` blocks (at most ``max_rows`` of them):
     their blocks of <= ``max_slots`` diagonals move to the mother-major dense

@@ -36,6 +36,7 @@ from .layout import (
     build_block_layout,
     build_tasks,
     build_wide_tasks,
+    build_chunked_tasks,
     dense_split,
     wide_shapes_ok,
     fd_tables,
@@ -257,12 +258,28 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         return 8 if (K >= 64 or K < 8) else 16
 
     def dense_rows_for(self, K):
-        """Use the mother-major pass for the dense rows: on for K < 8, where the
-        dense rows set the stage's critical path (plain nuclear solve 190 ->
-        115 ms with 4-mother chunks); off from K = 8 on, where the pass's extra
-        traffic costs more than it saves (167 vs 152 us per stage at K = 512)."""
+        """How the dense rows (the few rows with a block from almost every
+        species) are applied: "pass" = a mother-major kernel before the stage,
+        "chunks" = cut into one-wave mother-chunk tasks inside the stage kernel
+        plus a small epilogue launch, or None = ordinary split rows.  While
+        the right-hand sides fill few workgroups the dense rows' long
+        workgroups set the stage's critical path and "chunks" wins: plain
+        nuclear solve 18.4 us per stage vs 22.1 ("pass") and 43.6 (split
+        rows); K = 64: 57 vs 98 us.  At K = 512 the grid is full and split rows
+        are faster (137 vs 165 us: the chunk tasks' partial sums and extra
+        launch cost more than they save), so from K = 256 on the default is
+        split rows; the autotuner tries both.  A tuned True means "pass"."""
         tuned = getattr(self, "_tuned", {}).get(("dense", K))
-        return bool(tuned) if tuned is not None else K < 8
+        if tuned is not None:
+            return "pass" if tuned is True else (tuned or None)
+        return "chunks" if K < 256 else None
+
+    def dense_chunk_cost_for(self, K):
+        """Cost units per mother chunk of a dense row in the "chunks" mode
+        (24 below K = 8: 18.4 us per plain stage vs 22.6 with 12 and 19.5 with
+        48)."""
+        tuned = getattr(self, "_tuned", {}).get(("dense_chunk_cost", K))
+        return tuned if tuned is not None else (24 if K < 8 else 96)
 
     def kt_for(self, K):
         """Right-hand sides per wave: the largest of 8, 4, 2, 1 not above K."""
@@ -554,15 +571,25 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             wtasks, wheavy, nwslots = build_wide_tasks(wl.fwd, self.wide_part_cost_for(K))
         else:
             wgw = self.wg_waves_for(K, with_adjoint)
-            dense = None
+            dense = chunked = None
             split_fields = self._split_fields() is not None
-            if self.dense_rows_for(K) and not with_adjoint and not split_fields and self.kt_for(K) <= 8:
-                dense = dense_split(wl.blocks, wl.fwd.brow_ptr, nspec)
+            mode = self.dense_rows_for(K)
+            if mode and not with_adjoint and not split_fields and self.kt_for(K) <= 8:
+                if mode == "pass":
+                    dense = dense_split(wl.blocks, wl.fwd.brow_ptr, nspec)
+                elif mode == "chunks":
+                    chunked = build_chunked_tasks(
+                        wl.fwd, self.part_cost_for(K), wgw, chunk_cost=self.dense_chunk_cost_for(K),
+                        place=getattr(self, "_tuned", {}).get(("dense_place", K), "mother"))
+                else:
+                    raise ValueError(f"unknown dense-row mode {mode!r}")
             if dense is not None:
                 from dataclasses import replace as _dc_replace
 
                 rest = _dc_replace(wl.fwd, blocks=dense[3], brow_ptr=dense[4])
                 tasks = build_tasks(rest, self.part_cost_for(K), wgw)
+            elif chunked is not None:
+                tasks = chunked[0]
             else:
                 tasks = build_tasks(wl.fwd, self.part_cost_for(K), wgw)
         kt = self.kt_for(K)
@@ -585,7 +612,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         if has_src:
             flags |= _lib.PRH_F_INJECTION
         if wide:
-            dense = None
+            dense = chunked = None
         blocks = wl.blocks if dense is None else dense[3]
         blocks = blocks if blocks.size else np.zeros((1, 4))
         sshape = (nspec, d, K) if wide else (nspec, K, d)
@@ -618,6 +645,11 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             bufs["dense_row"] = dv(dense[1], np.int32)
             bufs["dense_tab"] = dv(dense[2], np.int32)
             bufs["dpart"] = torch.empty((ndchunk, len(dense[0]), K, d), dtype=torch.float64, device=dev)
+        if chunked is not None:
+            _, fin_tasks, drows, row_of, ndchunk = chunked
+            bufs["dense_row"] = dv(row_of, np.int32)
+            bufs["fin_tasks"] = dv(fin_tasks, np.int32)
+            bufs["dpart"] = torch.empty((ndchunk, len(drows), K, d), dtype=torch.float64, device=dev)
         if wide:
             bufs["wtasks"] = dv(wtasks, np.int32)
             bufs["wheavy"] = dv(wheavy if wheavy.size else np.zeros((1, 3)), np.int32)
@@ -649,10 +681,13 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
                 nheavy=int(wheavy.shape[0]) if wide_op else 0, pad2_=0,
                 wheavy=bufs["wheavy"].data_ptr() if wide_op else None,
                 wg_waves=wgw,
-                ndense=len(dense[0]) if (dense is not None and dense_op) else 0,
+                ndense=(len(dense[0]) if dense is not None else len(chunked[2]) if chunked is not None
+                        else 0) if dense_op else 0,
                 dchunk=DENSE_CHUNK, ndchunk=ndchunk,
-                dense_row=bufs["dense_row"].data_ptr() if (dense is not None and dense_op) else None,
+                dense_row=bufs["dense_row"].data_ptr() if ("dense_row" in bufs and dense_op) else None,
                 dense_tab=bufs["dense_tab"].data_ptr() if (dense is not None and dense_op) else None,
+                nfin=int(chunked[1].shape[0]) if (chunked is not None and dense_op) else 0, pad3_=0,
+                fin_tasks=bufs["fin_tasks"].data_ptr() if (chunked is not None and dense_op) else None,
             )
 
         op = make_op(bufs["tasks"], bufs["blocks"], bufs["dcoef"], wide_op=wide, wgw=fwd_wgw,
@@ -679,7 +714,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             rec_fa=None, rec_first=0, rec_count=0,
             wpart=bufs["wpart"].data_ptr() if wide else None,
             em=ctypes.addressof(em_struct) if em_struct is not None else None,
-            dpart=bufs["dpart"].data_ptr() if dense is not None else None,
+            dpart=bufs["dpart"].data_ptr() if "dpart" in bufs else None,
         )
         self._plan = {
             "op": op, "opT": opT, "sch": sch, "st": st, "x": x, "bufs": bufs,
@@ -687,7 +722,9 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             "stream": stream, "stride": stride, "dph": cache.dph, "K": K,
             "fold_rows": stride, "graph": None, "two": two, "z_grid": z_grid,
             "src_slots": slots, "J": J, "win": win, "wide": wide,
-            "dense_rows": list(dense[0]) if dense is not None else [],
+            "dense_rows": (list(dense[0]) if dense is not None
+                           else list(chunked[2]) if chunked is not None else []),
+            "dense_mode": "pass" if dense is not None else "chunks" if chunked is not None else None,
             "em": em_struct, "em_bufs": em_bufs,
         }
         if wide:
@@ -791,14 +828,14 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
     def tuning_candidates(self, K):
         """Stage configurations the autotuner times for K right-hand sides:
         right-hand sides per wave (kt), part budget (block cost per wave part,
-        i.e. rows per wave), waves per workgroup, and the wide layout when it
-        applies."""
+        i.e. rows per wave), waves per workgroup, the dense-row mode, and the
+        wide layout when it applies."""
         kts = [kt for kt in (8, 4, 2, 1) if kt <= K][:2]
         out = []
         for kt in kts:
             for pc in (48, 96) if K >= 8 else (12, 24):
                 for wgw in (8, 16):
-                    for dense in ((False,) if K >= 8 else (True, False)):
+                    for dense in (("chunks", False) if K >= 8 else (True, "chunks", False)):
                         out.append({"layout": "narrow", ("kt", K): kt, ("part_cost", K): pc,
                                     ("wg_waves", K): wgw, ("dense", K): dense})
         saved = getattr(self, "_tuned", None)

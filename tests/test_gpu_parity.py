@@ -155,17 +155,19 @@ def test_wide_layout_matches_narrow(part_cost):
         assert nb > 100 and dev <= TOL, (k, dev)
 
 
-def test_dense_rows_pass_matches_oracle_and_plain_tasks():
-    """K = 64 narrow with the mother-major pass for the dense rows (protons,
-    He-4) against the oracle columns and against the same solve without it.
-    z 0.2 -> 0: at z 0.1 one of these columns still holds a near-cancelling bin
-    (2.3e-8 for both kernels alike, see the EM test's note)."""
+@pytest.mark.parametrize("mode", ["pass", "chunks"])
+def test_dense_rows_pass_matches_oracle_and_plain_tasks(mode):
+    """K = 64 narrow with the dense rows (protons, He-4) applied by the
+    mother-major pass or as mother-chunk tasks, against the oracle columns and
+    against the same solve with ordinary split rows.  z 0.2 -> 0: at z 0.1 one
+    of these columns still holds a near-cancelling bin (2.3e-8 for both kernels
+    alike, see the EM test's note)."""
     run = workloads.nuclear_run()
     K = 64
     srcs = workloads.batch_sources(run, K, seed=8)
-    on = _multi(run, 0.2, K, srcs, {("dense", K): True})
+    on = _multi(run, 0.2, K, srcs, {("dense", K): mode})
     on.solve(dz=1e-3)
-    assert len(on._plan["dense_rows"]) == 2
+    assert len(on._plan["dense_rows"]) == 2 and on._plan["dense_mode"] == mode
     off = _multi(run, 0.2, K, srcs, {("dense", K): False})
     off.solve(dz=1e-3)
     assert off._plan["dense_rows"] == []
@@ -179,7 +181,7 @@ def test_dense_rows_pass_matches_oracle_and_plain_tasks():
         dev2, _ = rel_dev(on.state[:, k], off.state[:, k])
         assert nb > 100
         worst, worst_oo = max(worst, dev), max(worst_oo, dev2)
-    print(f"dense pass: vs oracle {worst:.3e}, vs ordinary tasks {worst_oo:.3e}")
+    print(f"dense rows ({mode}): vs oracle {worst:.3e}, vs ordinary tasks {worst_oo:.3e}")
     assert worst <= TOL and worst_oo <= TOL
 
 

@@ -171,3 +171,77 @@ def test_dense_split_keeps_the_product():
     scale = (np.abs(Moff) @ np.abs(X.reshape(-1))).reshape(nspec, d)
     assert np.all(np.abs(got - ref) <= 1e-13 * np.maximum(scale, 1e-300))
     assert row_of[rows[0]] == 0 and (row_of >= 0).sum() == len(rows)
+
+
+def test_chunked_dense_tasks_keep_the_product():
+    """Dense rows (protons, He-4) cut into mother-chunk tasks: emulating the
+    narrow stage kernel's task semantics (chunk tasks store dpart[chunk][row],
+    the epilogue launch adds each part's share of chunks, then the parts in
+    order) gives the off-diagonal product; every block, chunk slot and
+    workgroup position is checked."""
+    from prince_amd.solvers.layout import TASK_TO_DPART, build_chunked_tasks
+
+    run = workloads.nuclear_run()
+    lay, m0, M = _layout(run)
+    wl = WindowLayout(lay, None)
+    mv = _window_values(wl, m0)
+    nspec, d = lay.nspec, lay.d
+    wgw = 8
+    tasks, fin, rows, row_of, nch = build_chunked_tasks(lay, 96, wgw)
+    pdg = [run.spec_man.species_refs[r].pdgid for r in rows]
+    assert 2212 in pdg and 1000020040 in pdg
+    assert tasks.shape[0] % wgw == 0 and fin.shape[0] % wgw == 0
+    assert (row_of >= 0).sum() == len(rows) and nch >= 2
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((nspec, d))
+
+    def part_sum(b0, b1):
+        acc = np.zeros(d)
+        for b in range(b0, b1):
+            mo, olo, ns, voff = (int(v) for v in wl.blocks[b])
+            assert voff + ns * d <= mv.size
+            for s2 in range(ns):
+                for i in range(d):
+                    j = i + olo + s2
+                    if 0 <= j < d:
+                        acc[i] += mv[voff + s2 * d + i] * X[mo, j]
+        return acc
+
+    covered = np.zeros(wl.blocks.shape[0], dtype=np.int64)
+    dpart = np.full((nch, len(rows), d), np.nan)
+    got = np.full((nspec, d), np.nan)
+    red = {}
+    for t, (da, b0, b1, lead, nparts, kind) in enumerate(tasks):
+        if da < 0:
+            continue
+        covered[b0:b1] += 1
+        acc = part_sum(b0, b1)
+        if kind & TASK_TO_DPART:
+            r = row_of[da]
+            assert r >= 0 and nparts == 1 and 0 <= lead < nch
+            assert np.isnan(dpart[lead, r, 0])
+            dpart[lead, r] = acc
+            continue
+        assert row_of[da] < 0
+        wg, wv = divmod(t, wgw)
+        assert lead <= wv < lead + nparts <= wgw
+        red.setdefault((wg, lead), []).append(acc)
+        if wv == lead + nparts - 1:
+            got[da] = np.sum(red[(wg, lead)], axis=0)
+    assert np.all(covered == 1)
+    assert not np.isnan(dpart).any()
+    for t, (da, b0, b1, lead, nparts, kind) in enumerate(fin):
+        if da < 0:
+            continue
+        assert b0 == b1 and kind == 0 and row_of[da] >= 0
+        wg, wv = divmod(t, wgw)
+        assert lead <= wv < lead + nparts <= wgw
+        p = wv - lead
+        share = dpart[p * nch // nparts:(p + 1) * nch // nparts, row_of[da]].sum(axis=0)
+        red.setdefault(("fin", wg, lead), []).append(share)
+        if p == nparts - 1:
+            got[da] = np.sum(red[("fin", wg, lead)], axis=0)
+    Moff = M - sp.diags(M.diagonal())
+    ref = (Moff @ X.reshape(-1)).reshape(nspec, d)
+    scale = (np.abs(Moff) @ np.abs(X.reshape(-1))).reshape(nspec, d)
+    assert np.all(np.abs(got - ref) <= 1e-13 * np.maximum(scale, 1e-300))
