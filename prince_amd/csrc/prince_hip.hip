@@ -159,8 +159,9 @@ enum PartKind { PART_DIAG = 0, PART_BAND = 1, PART_GENERIC = 2 };
 
 // Coupling product of one row part over its blocks, in column order:
 //   accC[kk] = sum_b mC_b (shifted) y_b,   accE[kk] = sum_b mE_b (shifted) y_b
-// for the KT right-hand sides k0 .. k0+KT-1 (accE only with TWO).
-template <int KT, bool TWO>
+// for the KT right-hand sides k0 .. k0+KT-1 (accE only with TWO).  BACK adds
+// the DPP walk for bands left of the main diagonal (the transposed operator).
+template <int KT, bool TWO, bool BACK = false>
 __device__ __forceinline__ void couple_part(const prh_operator& op, const double* __restrict__ y,
                                             const double* __restrict__ mvC,
                                             const double* __restrict__ mvE, int b_begin,
@@ -276,6 +277,29 @@ __device__ __forceinline__ void couple_part(const prh_operator& op, const double
               if (s2 + 1 < ns[u]) {
 #pragma unroll
                 for (int kk = 0; kk < KT; ++kk) xv[u][kk] = lane_next(xv[u][kk]);
+              }
+            }
+          }
+        } else if (BACK && olo[u] + ns[u] <= 1) {
+          // every diagonal at or left of the main one (transposed bands):
+          // shuffle to the last diagonal, then walk back with wave_shr:1
+          const int top = olo[u] + ns[u] - 1;
+          if (top != 0) {
+#pragma unroll
+            for (int kk = 0; kk < KT; ++kk) xv[u][kk] = shfl_or_zero(xv[u][kk], lane + top, d);
+          }
+#pragma unroll
+          for (int t = 0; t < NSMAX; ++t) {
+            const int s2 = NSMAX - 1 - t;
+            if (s2 < ns[u]) {
+#pragma unroll
+              for (int kk = 0; kk < KT; ++kk) {
+                accC[kk] += mc[u][s2] * xv[u][kk];
+                if (TWO) accE[kk] += me[u][TWO ? s2 : 0] * xv[u][kk];
+              }
+              if (s2 > 0) {
+#pragma unroll
+                for (int kk = 0; kk < KT; ++kk) xv[u][kk] = lane_prev(xv[u][kk]);
               }
             }
           }
@@ -703,7 +727,8 @@ __global__ void __launch_bounds__(STHREADS) etd2_adjoint(prh_operator op, prh_sc
 
   double acc[KT], accE[KT];
   if (r >= 0) {
-    couple_part<KT, TWO>(op, yin, mvC, mvE, tk[1], tk[2], tk[5], k0, lane, active, acc, accE);
+    couple_part<KT, TWO, true>(op, yin, mvC, mvE, tk[1], tk[2], tk[5], k0, lane, active, acc,
+                               accE);
   } else {
 #pragma unroll
     for (int kk = 0; kk < KT; ++kk) acc[kk] = accE[kk] = 0.0;
@@ -739,10 +764,18 @@ __global__ void __launch_bounds__(STHREADS) etd2_adjoint(prh_operator op, prh_sc
     // transposed FD stencil D^T y at this row, then kappa-scaled
     double q = 0.0;
     if (pdiff) {
+      // neighbours yown[lane + o], o = -3..3 (zero outside [0, d)), added in
+      // the order o = -3, -2, -1, 1, 2, 3
+      const double ym1 = lane_prev(yown);
+      const double ym2 = lane_prev(ym1);
+      const double ym3 = lane_prev(ym2);
+      if (dcf[0] != 0.0) q += dcf[0] * ym3;
+      if (dcf[1] != 0.0) q += dcf[1] * ym2;
+      if (dcf[2] != 0.0) q += dcf[2] * ym1;
+      double yo = yown;
 #pragma unroll
-      for (int o = -3; o <= 3; ++o) {
-        if (o == 0) continue;
-        const double yo = shfl_or_zero(yown, lane + o, d);
+      for (int o = 1; o <= 3; ++o) {
+        yo = lane_next(yo);
         if (dcf[o + 3] != 0.0) q += dcf[o + 3] * yo;
       }
     }
@@ -1482,15 +1515,38 @@ __global__ void __launch_bounds__(STHREADS) etd2_uneven(prh_operator op, prh_une
         const int k = k0 + kk;
         y[kk] = (active && wk[kk] >= 0) ? xin[((int64_t)mo * K + k) * d + lane] : 0.0;
       }
-      for (int s2 = 0; s2 < ns; ++s2) {
-        const int src = lane + olo + s2;
+      if (olo >= 0) {
+        // diagonal s2 reads lane + olo + s2: one shuffle to the first
+        // diagonal (none when olo = 0), then one-lane DPP shifts
+        if (olo != 0) {
 #pragma unroll
-        for (int kk = 0; kk < KT; ++kk) {
-          const int k = k0 + kk;
-          const double m = (active && wk[kk] >= 0)
-                               ? un.mrhs[(int64_t)k * stride + voff + (int64_t)s2 * d + lane]
-                               : 0.0;
-          acc[kk] += m * shfl_or_zero(y[kk], src, d);
+          for (int kk = 0; kk < KT; ++kk) y[kk] = shfl_or_zero(y[kk], lane + olo, d);
+        }
+        for (int s2 = 0; s2 < ns; ++s2) {
+#pragma unroll
+          for (int kk = 0; kk < KT; ++kk) {
+            const int k = k0 + kk;
+            const double m = (active && wk[kk] >= 0)
+                                 ? un.mrhs[(int64_t)k * stride + voff + (int64_t)s2 * d + lane]
+                                 : 0.0;
+            acc[kk] += m * y[kk];
+          }
+          if (s2 + 1 < ns) {
+#pragma unroll
+            for (int kk = 0; kk < KT; ++kk) y[kk] = lane_next(y[kk]);
+          }
+        }
+      } else {
+        for (int s2 = 0; s2 < ns; ++s2) {
+          const int src = lane + olo + s2;
+#pragma unroll
+          for (int kk = 0; kk < KT; ++kk) {
+            const int k = k0 + kk;
+            const double m = (active && wk[kk] >= 0)
+                                 ? un.mrhs[(int64_t)k * stride + voff + (int64_t)s2 * d + lane]
+                                 : 0.0;
+            acc[kk] += m * shfl_or_zero(y[kk], src, d);
+          }
         }
       }
     }
