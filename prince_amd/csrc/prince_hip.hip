@@ -43,6 +43,14 @@ int32_t check_hip(hipError_t e, const char* what) {
 }
 
 constexpr int WAVE = 64;
+
+// Index of this wave in its workgroup, as a wave-uniform (scalar) value: the
+// compiler cannot prove threadIdx.x / 64 uniform by itself, and values read
+// through it (task fields, block ranges) would otherwise live in vector
+// registers behind exec-masked branches.
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+}
 constexpr int WAVES_PER_BLOCK = 4;
 constexpr int THREADS = WAVE * WAVES_PER_BLOCK;
 
@@ -69,7 +77,7 @@ __global__ void __launch_bounds__(THREADS) fold_gemm_mfma(const double* __restri
                                                           int nwin, double* __restrict__ out,
                                                           int64_t out_stride) {
   const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t wave = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x / WAVE);
+  const int64_t wave = (int64_t)blockIdx.x * WAVES_PER_BLOCK + wave_id();
   const int64_t r0 = wave * 16;
   if (r0 >= nrows) return;
   const int col = lane & 15;  // row of the kernel tile / column of D
@@ -143,6 +151,31 @@ __device__ __forceinline__ double lane_shift1(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 __device__ __forceinline__ double lane_next(double v) { return lane_shift1<DPP_WAVE_SHL1>(v); }
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// Buffer resource over `bytes` bytes from base (raw, no swizzle) and 64-bit
+// loads / stores at a per-lane byte offset voff plus a wave-uniform soff: the
+// address arithmetic stays in scalar registers.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const double* base, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0, bytes, 0x00020000);
+}
+__device__ __forceinline__ double tload(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void tstore(double v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+
+// Load / store of a double at a wave-uniform base plus a per-lane 32-bit byte
+// offset: addressed as SGPR base + VGPR offset, with no 64-bit address
+// arithmetic per access on the vector unit.
+__device__ __forceinline__ double ldu(const double* base, uint32_t byte_off) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+__device__ __forceinline__ void stu(double* base, uint32_t byte_off, double v) {
+  *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
 __device__ __forceinline__ double lane_prev(double v) { return lane_shift1<DPP_WAVE_SHR1>(v); }
 
 // Blocks whose loads are issued together before any of them is used: runs
@@ -169,8 +202,25 @@ __device__ __forceinline__ void couple_part(const prh_operator& op, const double
                                             double (&accC)[KT], double (&accE)[KT]) {
   const int K = op.K;
   const int d = op.d;
+  const int64_t Kd = (int64_t)K * d;
+  // lanes past the last bin read the last bin (every load in bounds, no
+  // branch); diagonal blocks leave their products in those lanes, band blocks
+  // zero them before lanes mix
+  const int lb = (active ? lane : d - 1) * 8;
+  // the window's coupling values and each mother's [K][d] panel as buffers;
+  // value offsets and right-hand-side columns (clamped to K - 1) go in the
+  // scalar offset
+  // buffer loads for the one-field forward stages; the two-field and adjoint
+  // kernels keep plain loads (the descriptors' scalar registers push those
+  // kernels into spills)
+  constexpr bool BUF = !TWO && !BACK;
+  const int vbytes = (int)(op.mvals_stride * 8);
+  const auto rmC = tile_rsrc(mvC, vbytes);
+  const int pbytes = (int)(Kd * 8);
+  int coff[KT];
 #pragma unroll
   for (int kk = 0; kk < KT; ++kk) {
+    coff[kk] = (k0 + kk < K ? k0 + kk : K - 1) * d * 8;
     accC[kk] = 0.0;
     accE[kk] = 0.0;
   }
@@ -202,21 +252,31 @@ __device__ __forceinline__ void couple_part(const prh_operator& op, const double
           mo = op.blocks[4 * bb];
           voffi = op.blocks[4 * bb + 3];
         }
-        const int64_t voff = voffi;
-        mc[u] = (active && valid) ? mvC[voff + lane] : 0.0;
-        me[u] = (TWO && active && valid) ? mvE[voff + lane] : 0.0;
+        if constexpr (BUF) {
+          mc[u] = tload(rmC, lb, voffi * 8);
+          const auto ry = tile_rsrc(y + (int64_t)mo * Kd, pbytes);
 #pragma unroll
-        for (int kk = 0; kk < KT; ++kk) {
-          const int k = k0 + kk;
-          xv[u][kk] = (active && k < K) ? y[((int64_t)mo * K + k) * d + lane] : 0.0;
+          for (int kk = 0; kk < KT; ++kk) xv[u][kk] = tload(ry, lb, coff[kk]);
+          me[u] = 0.0;
+        } else {
+          const int64_t voff = voffi;
+          mc[u] = active ? mvC[voff + lane] : 0.0;
+          me[u] = (TWO && active) ? mvE[voff + lane] : 0.0;
+#pragma unroll
+          for (int kk = 0; kk < KT; ++kk) {
+            const int k = k0 + kk;
+            xv[u][kk] = (active && k < K) ? y[((int64_t)mo * K + k) * d + lane] : 0.0;
+          }
         }
       }
 #pragma unroll
       for (int u = 0; u < UB; ++u) {
+        if (b + u < b_end) {
 #pragma unroll
-        for (int kk = 0; kk < KT; ++kk) {
-          accC[kk] += mc[u] * xv[u][kk];
-          if (TWO) accE[kk] += me[u] * xv[u][kk];
+          for (int kk = 0; kk < KT; ++kk) {
+            accC[kk] += mc[u] * xv[u][kk];
+            if (TWO) accE[kk] += me[u] * xv[u][kk];
+          }
         }
       }
     }
@@ -244,17 +304,31 @@ __device__ __forceinline__ void couple_part(const prh_operator& op, const double
         }
         olo[u] = ol;
         ns[u] = valid ? nsl : 0;
-        const int64_t voff = voffi;
+        if constexpr (BUF) {
 #pragma unroll
-        for (int s2 = 0; s2 < NSMAX; ++s2) {
-          const int sc = s2 < nsl ? s2 : nsl - 1;
-          mc[u][s2] = active ? mvC[voff + (int64_t)sc * d + lane] : 0.0;
-          if (TWO) me[u][TWO ? s2 : 0] = active ? mvE[voff + (int64_t)sc * d + lane] : 0.0;
-        }
+          for (int s2 = 0; s2 < NSMAX; ++s2) {
+            const int sc = s2 < nsl ? s2 : nsl - 1;
+            mc[u][s2] = tload(rmC, lb, (voffi + sc * d) * 8);
+          }
+          const auto ry = tile_rsrc(y + (int64_t)mo * Kd, pbytes);
 #pragma unroll
-        for (int kk = 0; kk < KT; ++kk) {
-          const int k = k0 + kk;
-          xv[u][kk] = (active && k < K) ? y[((int64_t)mo * K + k) * d + lane] : 0.0;
+          for (int kk = 0; kk < KT; ++kk) {
+            const double v = tload(ry, lb, coff[kk]);
+            xv[u][kk] = active ? v : 0.0;
+          }
+        } else {
+          const int64_t voff = voffi;
+#pragma unroll
+          for (int s2 = 0; s2 < NSMAX; ++s2) {
+            const int sc = s2 < nsl ? s2 : nsl - 1;
+            mc[u][s2] = active ? mvC[voff + (int64_t)sc * d + lane] : 0.0;
+            if (TWO) me[u][TWO ? s2 : 0] = active ? mvE[voff + (int64_t)sc * d + lane] : 0.0;
+          }
+#pragma unroll
+          for (int kk = 0; kk < KT; ++kk) {
+            const int k = k0 + kk;
+            xv[u][kk] = (active && k < K) ? y[((int64_t)mo * K + k) * d + lane] : 0.0;
+          }
         }
       }
 #pragma unroll
@@ -496,7 +570,9 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
   }
   const bool record = st.rec_x != nullptr && ridx >= 0 && ridx < st.rec_count;
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = threadIdx.x / WAVE;
+  // the two-field kernels keep the vector form: with a scalar wave index the
+  // compiler pipelines their wider batches past the register budget (spills)
+  const int wv = TWO ? (int)(threadIdx.x / WAVE) : wave_id();
   const int K = op.K;
   const int d = op.d;
   int gx, gy;
@@ -658,16 +734,8 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
   }
 }
 
-// Minimum waves per SIMD asked of the compiler: 6 for the one-field,
-// 8-rhs, no-dense-row stage (80 VGPRs instead of 87 with the band DPP chain:
-// 120 / 140 vs 137 / 147 us per stage at K = 512), otherwise its own choice.
-constexpr int stage_min_waves(int kt, bool two, int dm) {
-  return (kt == 8 && !two && dm == DM_NONE) ? 6 : 1;
-}
-
 template <bool STAGE2, int KT, bool TWO, int WGW, int DM>
 __global__ void __launch_bounds__(WGW * WAVE)
-    __attribute__((amdgpu_waves_per_eu(stage_min_waves(KT, TWO, DM))))
     etd2_stage(prh_operator op, prh_schedule sch, prh_state st, int step_local, int use_ctl) {
   stage_body<STAGE2, KT, TWO, WGW, DM>(op, sch, st, op.tasks, op.ntasks, step_local, use_ctl);
 }
@@ -711,7 +779,7 @@ __global__ void __launch_bounds__(STHREADS) etd2_adjoint(prh_operator op, prh_sc
                                                          prh_adjoint adj, int step) {
   __shared__ double red[SWG][KT][WAVE];
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = threadIdx.x / WAVE;
+  const int wv = threadIdx.x / WAVE;  // vector form: see stage_body
   const int K = op.K;
   const int d = op.d;
   int gx, gy;
@@ -841,7 +909,7 @@ template <bool TWO>
 __global__ void __launch_bounds__(STHREADS) adjoint_seed(prh_operator op, prh_schedule sch,
                                                          prh_adjoint adj, int step) {
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = threadIdx.x / WAVE;
+  const int wv = wave_id();
   const int K = op.K;
   const int d = op.d;
   const int* tk = op.tasks + 6 * ((int64_t)blockIdx.x * SWG + wv);
@@ -893,7 +961,7 @@ __device__ __forceinline__ void wide_step(const prh_state& st, int step_local, i
 
 // wave -> (task index, rhs group); XCD-aware when the group count is a multiple of 8
 __device__ __forceinline__ void wide_coords(int ntask, int ngroup, int& task, int& g) {
-  const int wv = threadIdx.x / WAVE;
+  const int wv = wave_id();
   const int nwg_t = (ntask + WWAVES - 1) / WWAVES;
   const int L = blockIdx.x;
   int tw;
@@ -914,16 +982,6 @@ __device__ __forceinline__ void wide_coords(int ntask, int ngroup, int& task, in
 // Buffer access to one (species, 64-rhs group) tile: element (bin i, lane) at
 // byte offset i * K * 8 (scalar soffset) + lane * 8 (vector voffset) from a
 // uniform base, so no per-lane 64-bit address arithmetic is needed.
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t tile_rsrc(const double* base, int bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0, bytes, 0x00020000);
-}
-__device__ __forceinline__ double tload(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
-__device__ __forceinline__ void tstore(double v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
-}
 
 template <bool STAGE2>
 __device__ __forceinline__ void wide_epilogue(const prh_operator& op, const prh_schedule& sch,
@@ -1180,7 +1238,7 @@ __global__ void __launch_bounds__(256) em_bh_weights(prh_operator op, prh_schedu
   wide_step(st, step_local, use_ctl, step, live);
   if (!live) return;
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = threadIdx.x / WAVE;
+  const int wv = wave_id();
   const int K = op.K;
   const int d = op.d;
   const int ng = (K + WAVE - 1) / WAVE;
@@ -1217,7 +1275,7 @@ __global__ void __launch_bounds__(EM_THREADS) em_step_kernel(prh_operator op, pr
   wide_step(st, step_local, use_ctl, step, live);
   if (!live) return;
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = threadIdx.x / WAVE;
+  const int wv = wave_id();
   const int K = op.K;
   const int d = op.d;
   const int de = em.d_em;
@@ -1439,7 +1497,7 @@ __global__ void __launch_bounds__(THREADS) fold_rows_mfma(const double* __restri
                                                           int ncol, double* __restrict__ out,
                                                           int64_t out_stride) {
   const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t wave = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x / WAVE);
+  const int64_t wave = (int64_t)blockIdx.x * WAVES_PER_BLOCK + wave_id();
   const int64_t r0 = wave * 16;
   if (r0 >= nrows) return;
   const int col = lane & 15;
@@ -1477,7 +1535,7 @@ __global__ void __launch_bounds__(STHREADS) etd2_uneven(prh_operator op, prh_une
                                                         prh_state st, int step) {
   __shared__ double red[SWG][KT][WAVE];
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = threadIdx.x / WAVE;
+  const int wv = wave_id();
   const int K = op.K;
   const int d = op.d;
   int gx, gy;
@@ -1673,7 +1731,7 @@ __global__ void __launch_bounds__(256) dense_pass(prh_operator op, prh_schedule 
   const int K = op.K;
   const int d = op.d;
   const int ny = (K + KT - 1) / KT;
-  const int item = blockIdx.x * 4 + threadIdx.x / WAVE;
+  const int item = blockIdx.x * 4 + wave_id();
   if (item >= op.ndchunk * ny) return;
   const int c = item / ny;
   const int k0 = (item % ny) * KT;
@@ -1761,6 +1819,8 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
     return set_error(std::string(who) + ": kt = 16 is for one-field forward stages without recording");
   if ((int64_t)(op->ntasks / wg_waves_of(*op)) * ((op->K + op->kt - 1) / op->kt) > 0x7fffffff)
     return set_error(std::string(who) + ": grid too large");
+  if (op->mvals_stride * 8 > 0x7fffffff || (int64_t)op->K * op->d * 8 > 0x7fffffff)
+    return set_error(std::string(who) + ": a window's values or one species' [K][d] panel exceeds 2 GiB");
   if (op->wide) {
     if (op->K % 64 != 0 || op->d != 64)
       return set_error(std::string(who) + ": the wide layout needs K % 64 == 0 and d == 64");

@@ -282,11 +282,13 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         return tuned if tuned is not None else (24 if K < 8 else 96)
 
     def kt_for(self, K):
-        """Right-hand sides per wave: the largest of 8, 4, 2, 1 not above K."""
+        """Right-hand sides per wave: 8 from K = 256 on (K = 512: 109 / 118 vs
+        118 / 123 us per stage with 4), else the largest of 4, 2, 1 not above K
+        (K = 64: 38 / 39 vs 48 / 52 us with 8: twice the workgroups)."""
         tuned = getattr(self, "_tuned", {}).get(("kt", K))
         if tuned is not None:
             return tuned
-        for kt in (8, 4, 2, 1):
+        for kt in ((8, 4, 2, 1) if K >= 256 else (4, 2, 1)):
             if kt <= K:
                 return kt
         return 1
