@@ -660,6 +660,17 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
   const int slot = inj ? op.src_slot[da] : 0;
   const int J = op.nsrc;
   const int64_t nk = (int64_t)op.nspec * K * d;
+  // one-field kernels: this row's [K][d] panels of the stage arrays as
+  // buffers (column offset in the scalar offset, bin offset per lane)
+  constexpr bool EBUF = !TWO;
+  const int64_t rowoff = (int64_t)da * K * d;
+  const int pb = (int)((int64_t)K * d * 8);
+  const int lb = (active ? lane : d - 1) * 8;
+  const int lo = lane * 8;
+  const auto r_in = tile_rsrc(xin + rowoff, pb);
+  const auto r_a = tile_rsrc(st.a + rowoff, pb);
+  const auto r_f = tile_rsrc(st.fphi + rowoff, pb);
+  const auto r_x = tile_rsrc(st.x + rowoff, pb);
 
   // row-local inputs, loaded once (the stores below may alias them)
   RowIn ri{};
@@ -676,24 +687,32 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
     RowCoef rc = rc0;
     if (TWO && active) rc = row_coef_from<TWO>(ri, sc_[kk], pdiff);
     const int64_t own = ((int64_t)da * K + k) * d + lane;
-    const double xown = active ? xin[own] : 0.0;
+    const int co = k * d * 8;
+    double xown;
+    if constexpr (EBUF) {
+      const double v = tload(r_in, lb, co);
+      xown = active ? v : 0.0;
+    } else {
+      xown = active ? xin[own] : 0.0;
+    }
     double out = acc[kk];
     if (pdiff) {
       const double kx = rc.kap * xown;
       // stencil neighbours kx[lane + o], o = -3..3 (zero outside [0, d)),
-      // added in the order o = -3, -2, -1, 1, 2, 3
+      // added in the order o = -3, -2, -1, 1, 2, 3 (a zero coefficient adds
+      // an exact zero: no per-term test)
       const double km1 = lane_prev(kx);
       const double km2 = lane_prev(km1);
       const double km3 = lane_prev(km2);
       double dx = 0.0;
-      if (dcf[0] != 0.0) dx += dcf[0] * km3;
-      if (dcf[1] != 0.0) dx += dcf[1] * km2;
-      if (dcf[2] != 0.0) dx += dcf[2] * km1;
+      dx += dcf[0] * km3;
+      dx += dcf[1] * km2;
+      dx += dcf[2] * km1;
       double kxo = kx;
 #pragma unroll
       for (int o = 1; o <= 3; ++o) {
         kxo = lane_next(kxo);
-        if (dcf[o + 3] != 0.0) dx += dcf[o + 3] * kxo;
+        dx += dcf[o + 3] * kxo;
       }
       out = out + dx;
     }
@@ -716,8 +735,13 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
       double sc = phi * out;
       sc = sc * hh;
       a = a + sc;
-      st.a[own] = a;
-      st.fphi[own] = out;
+      if constexpr (EBUF) {
+        tstore(a, r_a, lo, co);
+        tstore(out, r_f, lo, co);
+      } else {
+        st.a[own] = a;
+        st.fphi[own] = out;
+      }
       if (record) {
         st.rec_x[ridx * nk + own] = xown;
         st.rec_a[ridx * nk + own] = a;
@@ -725,10 +749,13 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
       }
     } else {
       const double phi = TWO ? phi2_of(hd, e) : phi0;
-      double sc = out - st.fphi[own];
+      double sc = out - (EBUF ? tload(r_f, lo, co) : st.fphi[own]);
       sc = sc * hh;
       sc = sc * phi;
-      st.x[own] = xown + sc;
+      if constexpr (EBUF)
+        tstore(xown + sc, r_x, lo, co);
+      else
+        st.x[own] = xown + sc;
       if (record) st.rec_fa[ridx * nk + own] = out;
     }
   }
@@ -837,14 +864,14 @@ __global__ void __launch_bounds__(STHREADS) etd2_adjoint(prh_operator op, prh_sc
       const double ym1 = lane_prev(yown);
       const double ym2 = lane_prev(ym1);
       const double ym3 = lane_prev(ym2);
-      if (dcf[0] != 0.0) q += dcf[0] * ym3;
-      if (dcf[1] != 0.0) q += dcf[1] * ym2;
-      if (dcf[2] != 0.0) q += dcf[2] * ym1;
+      q += dcf[0] * ym3;
+      q += dcf[1] * ym2;
+      q += dcf[2] * ym1;
       double yo = yown;
 #pragma unroll
       for (int o = 1; o <= 3; ++o) {
         yo = lane_next(yo);
-        if (dcf[o + 3] != 0.0) q += dcf[o + 3] * yo;
+        q += dcf[o + 3] * yo;
       }
     }
     const double mty = TWO ? acc[kk] + s * accE[kk] : acc[kk];
@@ -1646,19 +1673,20 @@ __global__ void __launch_bounds__(STHREADS) etd2_uneven(prh_operator op, prh_une
     if (pdiff) {
       const double kx = kap * xown;
       // stencil neighbours kx[lane + o], o = -3..3 (zero outside [0, d)),
-      // added in the order o = -3, -2, -1, 1, 2, 3
+      // added in the order o = -3, -2, -1, 1, 2, 3 (a zero coefficient adds
+      // an exact zero: no per-term test)
       const double km1 = lane_prev(kx);
       const double km2 = lane_prev(km1);
       const double km3 = lane_prev(km2);
       double dx = 0.0;
-      if (dcf[0] != 0.0) dx += dcf[0] * km3;
-      if (dcf[1] != 0.0) dx += dcf[1] * km2;
-      if (dcf[2] != 0.0) dx += dcf[2] * km1;
+      dx += dcf[0] * km3;
+      dx += dcf[1] * km2;
+      dx += dcf[2] * km1;
       double kxo = kx;
 #pragma unroll
       for (int o = 1; o <= 3; ++o) {
         kxo = lane_next(kxo);
-        if (dcf[o + 3] != 0.0) dx += dcf[o + 3] * kxo;
+        dx += dcf[o + 3] * kxo;
       }
       out = out + dx;
     }
