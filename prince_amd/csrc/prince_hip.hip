@@ -210,12 +210,12 @@ __device__ __forceinline__ void couple_part(const prh_operator& op, const double
   // the window's coupling values and each mother's [K][d] panel as buffers;
   // value offsets and right-hand-side columns (clamped to K - 1) go in the
   // scalar offset
-  // buffer loads for the one-field forward stages; the two-field and adjoint
-  // kernels keep plain loads (the descriptors' scalar registers push those
-  // kernels into spills)
-  constexpr bool BUF = !TWO && !BACK;
+  // buffer loads wherever the wave index is scalar (the two-field kernels
+  // with more than 4 right-hand sides per wave keep plain loads)
+  constexpr bool BUF = !TWO || KT <= 4;
   const int vbytes = (int)(op.mvals_stride * 8);
   const auto rmC = tile_rsrc(mvC, vbytes);
+  const auto rmE = tile_rsrc(TWO ? mvE : mvC, vbytes);
   const int pbytes = (int)(Kd * 8);
   int coff[KT];
 #pragma unroll
@@ -254,10 +254,10 @@ __device__ __forceinline__ void couple_part(const prh_operator& op, const double
         }
         if constexpr (BUF) {
           mc[u] = tload(rmC, lb, voffi * 8);
+          me[u] = TWO ? tload(rmE, lb, voffi * 8) : 0.0;
           const auto ry = tile_rsrc(y + (int64_t)mo * Kd, pbytes);
 #pragma unroll
           for (int kk = 0; kk < KT; ++kk) xv[u][kk] = tload(ry, lb, coff[kk]);
-          me[u] = 0.0;
         } else {
           const int64_t voff = voffi;
           mc[u] = active ? mvC[voff + lane] : 0.0;
@@ -309,6 +309,7 @@ __device__ __forceinline__ void couple_part(const prh_operator& op, const double
           for (int s2 = 0; s2 < NSMAX; ++s2) {
             const int sc = s2 < nsl ? s2 : nsl - 1;
             mc[u][s2] = tload(rmC, lb, (voffi + sc * d) * 8);
+            if (TWO) me[u][TWO ? s2 : 0] = tload(rmE, lb, (voffi + sc * d) * 8);
           }
           const auto ry = tile_rsrc(y + (int64_t)mo * Kd, pbytes);
 #pragma unroll
@@ -570,9 +571,10 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
   }
   const bool record = st.rec_x != nullptr && ridx >= 0 && ridx < st.rec_count;
   const int lane = threadIdx.x & (WAVE - 1);
-  // the two-field kernels keep the vector form: with a scalar wave index the
-  // compiler pipelines their wider batches past the register budget (spills)
-  const int wv = TWO ? (int)(threadIdx.x / WAVE) : wave_id();
+  // the two-field kernels with 8 or 16 right-hand sides per wave keep the
+  // vector form: with a scalar wave index the compiler pipelines their wider
+  // batches past the register budget (spills)
+  const int wv = (TWO && KT > 4) ? (int)(threadIdx.x / WAVE) : wave_id();
   const int K = op.K;
   const int d = op.d;
   int gx, gy;
@@ -662,7 +664,7 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
   const int64_t nk = (int64_t)op.nspec * K * d;
   // one-field kernels: this row's [K][d] panels of the stage arrays as
   // buffers (column offset in the scalar offset, bin offset per lane)
-  constexpr bool EBUF = !TWO;
+  constexpr bool EBUF = !TWO || KT <= 4;
   const int64_t rowoff = (int64_t)da * K * d;
   const int pb = (int)((int64_t)K * d * 8);
   const int lb = (active ? lane : d - 1) * 8;
@@ -806,7 +808,7 @@ __global__ void __launch_bounds__(STHREADS) etd2_adjoint(prh_operator op, prh_sc
                                                          prh_adjoint adj, int step) {
   __shared__ double red[SWG][KT][WAVE];
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = threadIdx.x / WAVE;  // vector form: see stage_body
+  const int wv = (TWO && KT > 4) ? (int)(threadIdx.x / WAVE) : wave_id();  // see stage_body
   const int K = op.K;
   const int d = op.d;
   int gx, gy;
