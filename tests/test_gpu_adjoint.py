@@ -14,12 +14,15 @@ from .parity import rel_dev
 pytestmark = pytest.mark.gpu
 
 
-def test_two_field_columns_match_oracle_with_scaled_ebl():
+@pytest.mark.parametrize("kt", [None, 8])  # default (2 per wave) and 8 per wave, 5 of them idle
+def test_two_field_columns_match_oracle_with_scaled_ebl(kt):
     run = workloads.nuclear_run()
     K = 3
     srcs = workloads.batch_sources(run, K, seed=5)
     scales = np.array([0.6, 1.0, 1.37])
     g = MultiRHSPropagationSolverETD2(0.12, 0.0, run, K=K)
+    if kt is not None:
+        g._tuned = {("kt", K): kt}
     for k in range(K):
         g[k].add_source_class(srcs[k])
     g.set_field_scales(1, scales)

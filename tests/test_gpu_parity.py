@@ -65,6 +65,31 @@ def test_nuclear_short():
     assert dev <= TOL
 
 
+def test_plain_dense_row_modes_agree():
+    """K = 1: dense rows as mother chunks (the default), through the
+    mother-major pass and as ordinary split rows give the same solve, and the
+    default meets the oracle."""
+    run = workloads.nuclear_run()
+    states = {}
+    for mode in ("chunks", True, False):
+        g = UHECRPropagationSolverETD2(0.3, 0.0, run)
+        g._tuned = {("dense", 1): mode}
+        g.add_source_class(workloads.silicon_source(run))
+        g.solve(dz=1e-3)
+        states[str(mode)] = g.state.copy()
+        assert g._plan["dense_mode"] == {"chunks": "chunks", "True": "pass", "False": None}[str(mode)]
+    r = OracleETD2(0.3, 0.0, run)
+    r.add_source_class(workloads.silicon_source(run))
+    r.solve(dz=1e-3)
+    dev, nb = rel_dev(states["chunks"], r.state)
+    print(f"plain chunks vs oracle: {nb} bins, max rel dev {dev:.3e}")
+    assert nb > 100 and dev <= TOL
+    for mode in ("True", "False"):
+        d2, _ = rel_dev(states[mode], states["chunks"])
+        print(f"plain {mode} vs chunks: {d2:.3e}")
+        assert d2 <= 1e-9
+
+
 def test_multi_rhs_matches_columns():
     run = workloads.nuclear_run()
     K = 4
