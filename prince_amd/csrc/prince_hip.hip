@@ -848,6 +848,10 @@ __global__ void __launch_bounds__(STHREADS) etd2_adjoint(prh_operator op, prh_sc
   const bool want_s = TWO && adj.gdot != nullptr;
   const bool inj = !ASTAGE2 && adj.gsrc0 != nullptr && (op.flags & PRH_F_INJECTION) != 0 &&
                    op.src_slot[r] >= 0;
+  // row-local inputs of this step (and of the previous one for u), loaded once
+  RowIn ri{}, rip{};
+  if (active) ri = row_in<TWO>(op, sch, step, w, r, lane);
+  if (ASTAGE2 && step > 0 && active) rip = row_in<TWO>(op, sch, step - 1, sch.win[step - 1], r, lane);
 
 #pragma unroll
   for (int kk = 0; kk < KT; ++kk) {
@@ -855,7 +859,7 @@ __global__ void __launch_bounds__(STHREADS) etd2_adjoint(prh_operator op, prh_sc
     if (k >= K) break;
     const double s = TWO ? op.escale[k] : 0.0;
     RowCoef rc{0.0, 0.0, 0.0, 0.0};
-    if (active) rc = row_coef<TWO>(op, sch, step, w, r, lane, s, pdiff);
+    if (active) rc = row_coef_from<TWO>(ri, s, pdiff);
     const int64_t own = ((int64_t)r * K + k) * d + lane;
     const double yown = active ? yin[own] : 0.0;
     // transposed FD stencil D^T y at this row, then kappa-scaled
@@ -922,8 +926,7 @@ __global__ void __launch_bounds__(STHREADS) etd2_adjoint(prh_operator op, prh_sc
       // u of the previous step: h_{k-1} phi2(h_{k-1} d_{k-1}) lam
       if (step > 0) {
         const int sp = step - 1;
-        const int wp = sch.win[sp];
-        const RowCoef rp = row_coef<TWO>(op, sch, sp, wp, r, lane, s, pdiff);
+        const RowCoef rp = row_coef_from<TWO>(rip, s, pdiff);
         const double hdp = rp.dval * sch.h[sp];
         const double ep = exp(hdp);
         adj.u[own] = sch.h[sp] * phi2_of(hdp, ep) * lam;
