@@ -1297,6 +1297,45 @@ __global__ void __launch_bounds__(256) em_bh_weights(prh_operator op, prh_schedu
   em.wbh[(int64_t)j * K + k] = em.bhw[(int64_t)w * d + j] * tot;
 }
 
+// Narrow layout ([species][rhs][bin]): workgroup = one right-hand side, lanes
+// = bins, so every species panel is one coalesced 512-byte load; the 4 waves
+// take the species in the same interleaved quarters and order as
+// em_bh_weights and combine them in wave order (identical sums).
+__global__ void __launch_bounds__(256) em_bh_weights_bins(prh_operator op, prh_schedule sch,
+                                                          prh_state st, prh_em em,
+                                                          int step_local, int use_ctl) {
+  __shared__ double part[4][WAVE];
+  int step;
+  bool live;
+  wide_step(st, step_local, use_ctl, step, live);
+  if (!live) return;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = wave_id();
+  const int K = op.K;
+  const int d = op.d;
+  const int k = blockIdx.x;
+  const bool act = lane < d;
+  const int ns = op.nspec;
+  const int64_t sS = (int64_t)K * d;
+  const double* xp = st.x + (int64_t)k * d + (act ? lane : 0);
+  double acc = 0.0;
+  int s = wv;
+  for (; s + 28 < ns; s += 32) {
+    double xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = act ? xp[(int64_t)(s + 4 * u) * sS] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += em.cbh[s + 4 * u] * xv[u];
+  }
+  for (; s < ns; s += 4) acc += em.cbh[s] * (act ? xp[(int64_t)s * sS] : 0.0);
+  part[wv][lane] = acc;
+  __syncthreads();
+  if (wv != 0 || !act) return;
+  const double tot = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+  const int w = sch.win[step];
+  em.wbh[(int64_t)lane * K + k] = em.bhw[(int64_t)w * d + lane] * tot;
+}
+
 __global__ void __launch_bounds__(EM_THREADS) em_step_kernel(prh_operator op, prh_schedule sch,
                                                             prh_state st, prh_em em,
                                                             int step_local, int use_ctl) {
@@ -1445,8 +1484,12 @@ void launch_em(const prh_operator& op, const prh_schedule& sch, const prh_state&
     sK = d;
   }
   const int ng = (op.K + WAVE - 1) / WAVE;
-  hipLaunchKernelGGL(em_bh_weights, dim3((unsigned)(op.d * ng)), dim3(256), 0, s, op, sch, st, em,
-                     sS, sJ, sK, step_local, use_ctl);
+  if (op.wide)
+    hipLaunchKernelGGL(em_bh_weights, dim3((unsigned)(op.d * ng)), dim3(256), 0, s, op, sch, st,
+                       em, sS, sJ, sK, step_local, use_ctl);
+  else
+    hipLaunchKernelGGL(em_bh_weights_bins, dim3((unsigned)op.K), dim3(256), 0, s, op, sch, st, em,
+                       step_local, use_ctl);
   hipLaunchKernelGGL(em_step_kernel, dim3((unsigned)(ng * EM_RC)), dim3(EM_THREADS), 0, s, op, sch, st, em,
                      step_local, use_ctl);
 }
