@@ -151,6 +151,7 @@ __device__ __forceinline__ double lane_shift1(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 __device__ __forceinline__ double lane_next(double v) { return lane_shift1<DPP_WAVE_SHL1>(v); }
+__device__ __forceinline__ double lane_prev(double v) { return lane_shift1<DPP_WAVE_SHR1>(v); }
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
@@ -166,17 +167,6 @@ __device__ __forceinline__ double tload(__amdgpu_buffer_rsrc_t r, int voff, int 
 __device__ __forceinline__ void tstore(double v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
 }
-
-// Load / store of a double at a wave-uniform base plus a per-lane 32-bit byte
-// offset: addressed as SGPR base + VGPR offset, with no 64-bit address
-// arithmetic per access on the vector unit.
-__device__ __forceinline__ double ldu(const double* base, uint32_t byte_off) {
-  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + byte_off);
-}
-__device__ __forceinline__ void stu(double* base, uint32_t byte_off, double v) {
-  *reinterpret_cast<double*>(reinterpret_cast<char*>(base) + byte_off) = v;
-}
-__device__ __forceinline__ double lane_prev(double v) { return lane_shift1<DPP_WAVE_SHR1>(v); }
 
 // Blocks whose loads are issued together before any of them is used: runs
 // of diagonal blocks and of band blocks with at most NSMAX diagonals.  With
