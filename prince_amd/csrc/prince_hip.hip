@@ -672,6 +672,18 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
   const double hd0 = rc0.dval * hh;
   const double e0 = exp(hd0);
   const double phi0 = STAGE2 ? phi2_of(hd0, e0) : phi1_of(hd0, e0);
+  // the row's own state (and, in stage 2, F(x)) for all KT columns, issued
+  // together before the column loop
+  double xo[KT], fo[KT];
+  if constexpr (EBUF) {
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) {
+      const int co = (k0 + kk < K ? k0 + kk : K - 1) * d * 8;
+      const double v = tload(r_in, lb, co);
+      xo[kk] = active ? v : 0.0;
+      fo[kk] = STAGE2 ? tload(r_f, lb, co) : 0.0;
+    }
+  }
 #pragma unroll
   for (int kk = 0; kk < KT; ++kk) {
     const int k = k0 + kk;
@@ -682,8 +694,7 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
     const int co = k * d * 8;
     double xown;
     if constexpr (EBUF) {
-      const double v = tload(r_in, lb, co);
-      xown = active ? v : 0.0;
+      xown = xo[kk];
     } else {
       xown = active ? xin[own] : 0.0;
     }
@@ -741,7 +752,7 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
       }
     } else {
       const double phi = TWO ? phi2_of(hd, e) : phi0;
-      double sc = out - (EBUF ? tload(r_f, lo, co) : st.fphi[own]);
+      double sc = out - (EBUF ? fo[kk] : st.fphi[own]);
       sc = sc * hh;
       sc = sc * phi;
       if constexpr (EBUF)
