@@ -611,6 +611,32 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
       }
     }
   }
+  // one-field kernels: this row's [K][d] panels of the stage arrays as
+  // buffers (column offset in the scalar offset, bin offset per lane); a row's
+  // lead wave issues its own state (and, in stage 2, F(x)) for all KT columns
+  // before the part reduction so the loads overlap the wait
+  constexpr bool EBUF = !TWO || KT <= 4;
+  const bool erow = da >= 0 && wv == tk[3] && !(DM == DM_CHUNKS && (tk[5] & PRH_TASK_TO_DPART));
+  const int64_t rowoff = (int64_t)(da >= 0 ? da : 0) * K * d;
+  const int pb = (int)((int64_t)K * d * 8);
+  const int lb = (active ? lane : d - 1) * 8;
+  const int lo = lane * 8;
+  const auto r_in = tile_rsrc(xin + rowoff, pb);
+  const auto r_a = tile_rsrc(st.a + rowoff, pb);
+  const auto r_f = tile_rsrc(st.fphi + rowoff, pb);
+  const auto r_x = tile_rsrc(st.x + rowoff, pb);
+  double xo[KT], fo[KT];
+  if constexpr (EBUF) {
+    if (erow) {
+#pragma unroll
+      for (int kk = 0; kk < KT; ++kk) {
+        const int co = (k0 + kk < K ? k0 + kk : K - 1) * d * 8;
+        const double v = tload(r_in, lb, co);
+        xo[kk] = active ? v : 0.0;
+        fo[kk] = STAGE2 ? tload(r_f, lb, co) : 0.0;
+      }
+    }
+  }
   reduce_parts<KT>(red, wv, lane, tk[3], tk[4], acc);
   if (DM == DM_CHUNKS && da >= 0 && (tk[5] & PRH_TASK_TO_DPART)) {
     // one mother chunk of a dense row: store its partial sum (chunk tk[3])
@@ -652,17 +678,6 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
   const int slot = inj ? op.src_slot[da] : 0;
   const int J = op.nsrc;
   const int64_t nk = (int64_t)op.nspec * K * d;
-  // one-field kernels: this row's [K][d] panels of the stage arrays as
-  // buffers (column offset in the scalar offset, bin offset per lane)
-  constexpr bool EBUF = !TWO || KT <= 4;
-  const int64_t rowoff = (int64_t)da * K * d;
-  const int pb = (int)((int64_t)K * d * 8);
-  const int lb = (active ? lane : d - 1) * 8;
-  const int lo = lane * 8;
-  const auto r_in = tile_rsrc(xin + rowoff, pb);
-  const auto r_a = tile_rsrc(st.a + rowoff, pb);
-  const auto r_f = tile_rsrc(st.fphi + rowoff, pb);
-  const auto r_x = tile_rsrc(st.x + rowoff, pb);
 
   // row-local inputs, loaded once (the stores below may alias them)
   RowIn ri{};
@@ -672,18 +687,6 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
   const double hd0 = rc0.dval * hh;
   const double e0 = exp(hd0);
   const double phi0 = STAGE2 ? phi2_of(hd0, e0) : phi1_of(hd0, e0);
-  // the row's own state (and, in stage 2, F(x)) for all KT columns, issued
-  // together before the column loop
-  double xo[KT], fo[KT];
-  if constexpr (EBUF) {
-#pragma unroll
-    for (int kk = 0; kk < KT; ++kk) {
-      const int co = (k0 + kk < K ? k0 + kk : K - 1) * d * 8;
-      const double v = tload(r_in, lb, co);
-      xo[kk] = active ? v : 0.0;
-      fo[kk] = STAGE2 ? tload(r_f, lb, co) : 0.0;
-    }
-  }
 #pragma unroll
   for (int kk = 0; kk < KT; ++kk) {
     const int k = k0 + kk;
