@@ -152,6 +152,13 @@ __device__ __forceinline__ double lane_shift1(double v) {
 }
 __device__ __forceinline__ double lane_next(double v) { return lane_shift1<DPP_WAVE_SHL1>(v); }
 __device__ __forceinline__ double lane_prev(double v) { return lane_shift1<DPP_WAVE_SHR1>(v); }
+// value of lane l (wave-uniform l) as a scalar
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
@@ -1424,17 +1431,23 @@ __global__ void __launch_bounds__(EM_THREADS) em_step_kernel(prh_operator op, pr
     sc = sc * p2[t];
     xn[t] = ga[i * WAVE + lane] + sc;
   }
-  // Bethe-Heitler pairs: e_src = R_BH w, lep = e- + e+ = 2 e_src
-  double es[EM_NB];
-#pragma unroll
-  for (int t = 0; t < EM_NB; ++t) es[t] = 0.0;
+  // Bethe-Heitler pairs: e_src = R_BH w, lep = e- + e+ = 2 e_src.  The wave's
+  // rows of R_BH come in with one vector load each (lane = j) and are
+  // broadcast with readlane, instead of one scalar load per coefficient.
+  double es[EM_NB], Rr[EM_NB];
   const double* R = em.rbh + (int64_t)w * de * d;
+#pragma unroll
+  for (int t = 0; t < EM_NB; ++t) {
+    const int i = wv + EM_WAVES * t;
+    es[t] = 0.0;
+    Rr[t] = (i < de && lane < d) ? R[(int64_t)i * d + lane] : 0.0;
+  }
   for (int j = 0; j < d; ++j) {
     const double wj = act ? em.wbh[(int64_t)j * K + k] : 0.0;
 #pragma unroll
     for (int t = 0; t < EM_NB; ++t) {
       const int i = wv + EM_WAVES * t;
-      if (i < de) es[t] += R[(int64_t)i * d + j] * wj;
+      if (i < de) es[t] += readlane_f64(Rr[t], j) * wj;
     }
   }
   __syncthreads();  // every wave is done reading g0 and ga
@@ -1450,18 +1463,36 @@ __global__ void __launch_bounds__(EM_THREADS) em_step_kernel(prh_operator op, pr
   const double* Tg = em.tg + (int64_t)w * de * de;
   const double* Te = em.te + (int64_t)w * de * de;
   constexpr int NR = (PRH_EM_MAX / EM_RC + EM_WAVES) / EM_WAVES;
-  double ag[NR], ae[NR];
+  constexpr int NC = (PRH_EM_MAX + WAVE - 1) / WAVE;
+  // the wave's transfer rows with one vector load per 64 columns (lane = m
+  // within the slice), broadcast with readlane
+  double ag[NR], ae[NR], tg[NR][NC], te[NR][NC];
 #pragma unroll
-  for (int t = 0; t < NR; ++t) ag[t] = ae[t] = 0.0;
-  for (int m = 0; m < de; ++m) {
-    const double gm = g0[m * WAVE + lane];
-    const double lm = ga[m * WAVE + lane];
+  for (int t = 0; t < NR; ++t) {
+    const int i = r0 + wv + EM_WAVES * t;
+    ag[t] = ae[t] = 0.0;
 #pragma unroll
-    for (int t = 0; t < NR; ++t) {
-      const int i = r0 + wv + EM_WAVES * t;
-      if (i < r1) {
-        ag[t] += Tg[(int64_t)i * de + m] * gm;
-        ae[t] += Te[(int64_t)i * de + m] * lm;
+    for (int c = 0; c < NC; ++c) {
+      const int m = c * WAVE + lane;
+      const bool ok = i < r1 && m < de;
+      tg[t][c] = ok ? Tg[(int64_t)i * de + m] : 0.0;
+      te[t][c] = ok ? Te[(int64_t)i * de + m] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int mend = min(de - c * WAVE, WAVE);
+    for (int l = 0; l < mend; ++l) {
+      const int m = c * WAVE + l;
+      const double gm = g0[m * WAVE + lane];
+      const double lm = ga[m * WAVE + lane];
+#pragma unroll
+      for (int t = 0; t < NR; ++t) {
+        const int i = r0 + wv + EM_WAVES * t;
+        if (i < r1) {
+          ag[t] += readlane_f64(tg[t][c], l) * gm;
+          ae[t] += readlane_f64(te[t][c], l) * lm;
+        }
       }
     }
   }
