@@ -347,7 +347,7 @@ def main():
     if args.gpus != 1:
         print(
             "bench.py measures one GPU: the ETD2 path shards by instance with no per-step "
-            "exchange, and this build runs and reports the single-GPU share only.",
+            "exchange, and this benchmark runs and reports the single-GPU share only.",
             file=sys.stderr,
         )
         sys.exit(2)
