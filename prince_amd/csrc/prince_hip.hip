@@ -1442,12 +1442,22 @@ __global__ void __launch_bounds__(EM_THREADS) em_step_kernel(prh_operator op, pr
     es[t] = 0.0;
     Rr[t] = (i < de && lane < d) ? R[(int64_t)i * d + lane] : 0.0;
   }
-  for (int j = 0; j < d; ++j) {
-    const double wj = act ? em.wbh[(int64_t)j * K + k] : 0.0;
+  // the weights of 8 bins are loaded together before they are used
+  for (int j0 = 0; j0 < d; j0 += 8) {
+    double wj[8];
 #pragma unroll
-    for (int t = 0; t < EM_NB; ++t) {
-      const int i = wv + EM_WAVES * t;
-      if (i < de) es[t] += readlane_f64(Rr[t], j) * wj;
+    for (int u = 0; u < 8; ++u) {
+      const int j = j0 + u < d ? j0 + u : d - 1;
+      wj[u] = act ? em.wbh[(int64_t)j * K + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (j0 + u >= d) break;  // uniform
+#pragma unroll
+      for (int t = 0; t < EM_NB; ++t) {
+        const int i = wv + EM_WAVES * t;
+        if (i < de) es[t] += readlane_f64(Rr[t], j0 + u) * wj[u];
+      }
     }
   }
   __syncthreads();  // every wave is done reading g0 and ga
