@@ -863,6 +863,20 @@ __global__ void __launch_bounds__(STHREADS) etd2_adjoint(prh_operator op, prh_sc
   RowIn ri{}, rip{};
   if (active) ri = row_in<TWO>(op, sch, step, w, r, lane);
   if (ASTAGE2 && step > 0 && active) rip = row_in<TWO>(op, sch, step - 1, sch.win[step - 1], r, lane);
+  // up to 4 columns per wave: the row's own adjoint state and lam (stage 1) /
+  // lam_a (stage 2) for all columns, issued together before the column loop
+  // (with 8 the registers are spoken for)
+  constexpr bool PRE = KT <= 4;
+  double ypre[KT], lpre[KT];
+  if constexpr (PRE) {
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) {
+      const int kc = k0 + kk < K ? k0 + kk : K - 1;
+      const int64_t own = ((int64_t)r * K + kc) * d + lane;
+      ypre[kk] = active ? yin[own] : 0.0;
+      lpre[kk] = active ? (ASTAGE2 ? adj.lam_a[own] : adj.lam[own]) : 0.0;
+    }
+  }
 
 #pragma unroll
   for (int kk = 0; kk < KT; ++kk) {
@@ -872,7 +886,7 @@ __global__ void __launch_bounds__(STHREADS) etd2_adjoint(prh_operator op, prh_sc
     RowCoef rc{0.0, 0.0, 0.0, 0.0};
     if (active) rc = row_coef_from<TWO>(ri, s, pdiff);
     const int64_t own = ((int64_t)r * K + k) * d + lane;
-    const double yown = active ? yin[own] : 0.0;
+    const double yown = PRE ? ypre[kk] : (active ? yin[own] : 0.0);
     // transposed FD stencil D^T y at this row, then kappa-scaled
     double q = 0.0;
     if (pdiff) {
@@ -900,7 +914,7 @@ __global__ void __launch_bounds__(STHREADS) etd2_adjoint(prh_operator op, prh_sc
     const double p2 = phi2_of(hd, e);
     double gs = 0.0;
     if (!ASTAGE2) {
-      const double lamp = adj.lam[own];
+      const double lamp = PRE ? lpre[kk] : adj.lam[own];
       const double lam_a = lamp + t;
       const double wv_ = hh * p1 * lam_a;
       adj.lam_a[own] = lam_a;
@@ -927,7 +941,7 @@ __global__ void __launch_bounds__(STHREADS) etd2_adjoint(prh_operator op, prh_sc
         gs += dJdd * rc.dE;
       }
     } else {
-      const double lam_a = adj.lam_a[own];
+      const double lam_a = PRE ? lpre[kk] : adj.lam_a[own];
       const double lam = e * lam_a + t;
       adj.lam[own] = lam;
       if (want_s) {
