@@ -44,6 +44,7 @@ import scipy.sparse as sp
 from prince_amd.model.cosmology import H
 from prince_amd.model.partial_diff import DifferentialOperator
 from prince_amd.model.units import PRINCE_UNITS
+from prince_amd.solvers.run_inputs import fold_kernel_fp64
 
 PHI1_SMALL = 1e-6
 PHI2_SMALL = 1e-3
@@ -184,6 +185,7 @@ class OracleETD2:
         self.D_diag, self.D_off = split_operator(D.tocsr())
         self.state = None
         self.refresh_zs = []
+        self._B = None
         self.em = bool(getattr(prince_run, "enable_em_cascade", False))
         if self.em:
             from prince_amd.model.cascade import cascade_kernels
@@ -224,7 +226,9 @@ class OracleETD2:
 
     def _refresh(self, z):
         ir = self.run.int_rates
-        B = ir.batch_matrix
+        if self._B is None:
+            self._B = fold_kernel_fp64(ir)
+        B = self._B
         ph = ir.photon_vector(z, pfield=self.pfield)
         if self._M_off is None:
             M = ir.coupling_mat.copy()

@@ -303,6 +303,7 @@ class ETD2Propagate:
                 th = theta.detach().cpu().numpy()
                 solver, plan, st = problem.forward(th, with_adjoint=True)
                 ctx.problem, ctx.solver, ctx.plan, ctx.theta = problem, solver, plan, th
+                ctx.theta_device, ctx.theta_dtype = theta.device, theta.dtype
                 return st.detach().clone()
 
             @staticmethod
@@ -310,7 +311,7 @@ class ETD2Propagate:
                 lam_final = grad_state.permute(0, 2, 1).contiguous()
                 ab = ctx.solver.adjoint_sweep(ctx.plan, lam_final)
                 g = ctx.problem.contract(ctx.solver, ctx.plan, ab, ctx.theta)
-                return torch.as_tensor(g, device=grad_state.device), None
+                return torch.as_tensor(g, dtype=ctx.theta_dtype, device=ctx.theta_device), None
 
         cls._fn = _Fn
         return _Fn

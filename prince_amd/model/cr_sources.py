@@ -19,58 +19,77 @@ import numpy as np
 from .cosmology import star_formation_rate
 
 
+def _sfr_weighted(z, m):
+    return (1 + z) ** m * star_formation_rate(z)
+
+
+def _broken_at_one(z, m):
+    return (1 + z) ** m if z <= 1 else (1 + 1) ** m
+
+
+def _broken_at_one_to_sfr(z, m):
+    return (1 + z) ** m if z <= 1 else (1 + 1) ** 3.6 * (1 + z) ** (m - 3.6)
+
+
+#: (keyword, m) evolutions of the reference (cr_sources.py:110-162)
+EVOLUTIONS = {
+    "SFR": _sfr_weighted,
+    "TDE": lambda z, m: (1 + z) ** (m - 3.0),
+    "simple": lambda z, m: (1 + z) ** m,
+    "simple_flat": _broken_at_one,
+    "simple_SFR": _broken_at_one_to_sfr,
+}
+
+
+def evolution_factor(m, z):
+    """Source density evolution at z for the reference's evolution spec m:
+    "flat", a float (then (1+z)^m times the star-formation rate) or
+    (keyword, m) with a keyword of ``EVOLUTIONS``."""
+    if z < 0:
+        raise ValueError(f"source evolution is defined for z >= 0 only (z = {z})")
+    if m == "flat":
+        return 1.0
+    if isinstance(m, float):
+        return _sfr_weighted(z, m)
+    if isinstance(m, tuple) and len(m) == 2 and m[0] in EVOLUTIONS:
+        return EVOLUTIONS[m[0]](z, m[1])
+    raise ValueError(f"source evolution {m!r}: expected 'flat', a float or one of "
+                     f"{sorted(EVOLUTIONS)} with an index")
+
+
 class CosmicRaySource:
+    """Injection ``norm * evolution(z) * injection_grid`` (cr_sources.py:14-87).
+
+    ``params`` maps PDG ids to the spectrum parameters of the subclass;
+    ``injection_grid`` holds every injected species' spectrum in its block
+    of the state vector."""
+
     def __init__(self, prince_run, pdgids=None, params=None, norm=1.0, m="flat"):
+        self.spec_man = prince_run.spec_man
         self.cr_grid = prince_run.cr_grid.grid
         self.dim_states = prince_run.dim_states
-        self.norm = norm
-        self.spec_man = prince_run.spec_man
         self.params = params
-        self.pdgids = np.array(pdgids if pdgids is not None else list(params.keys()))
-        self.pdgids.sort()
+        self.norm = norm
         self.source_evo_m = m
-        self._compute_injection_grid()
+        ids = list(params.keys()) if pdgids is None else list(pdgids)
+        self.pdgids = np.sort(np.array(ids))
+        self.injection_grid = self._injection_grid()
 
-    def _compute_injection_grid(self):
-        self.injection_grid = np.zeros(self.dim_states)
-        for pid in self.pdgids:
-            params = self.params[pid]
-            inj_spec = self.spec_man.pdgid2sref[int(pid)]
-            self.injection_grid[inj_spec.sl] = self.injection_spectrum(
-                int(pid), self.cr_grid, params
-            )
+    def _injection_grid(self):
+        grid = np.zeros(self.dim_states)
+        for pid in (int(p) for p in self.pdgids):
+            block = self.spec_man.pdgid2sref[pid].sl
+            grid[block] = self.injection_spectrum(pid, self.cr_grid, self.params[pid])
+        return grid
 
     def injection_rate(self, z):
         return self.norm * self.evolution(z) * self.injection_grid
 
     def injection_spectrum(self, pid, energy, params):  # pragma: no cover
-        raise NotImplementedError
+        raise NotImplementedError("subclasses define the injection spectrum")
 
     def evolution(self, z):
-        if z < 0:
-            raise Exception("Source evolution not defined for negative z = {:}".format(z))
-        m = self.source_evo_m
-        if m == "flat":
-            return 1.0
-        elif type(m) is float:
-            return (1 + z) ** m * star_formation_rate(z)
-        elif type(m) is tuple:
-            mkwd, mval = m
-            if mkwd == "SFR":
-                return (1 + z) ** mval * star_formation_rate(z)
-            elif mkwd == "TDE":
-                return (1 + z) ** (mval - 3.0)
-            elif mkwd == "simple":
-                return (1 + z) ** mval
-            elif mkwd == "simple_flat":
-                if z <= 1:
-                    return (1 + z) ** mval
-                return (1 + 1) ** mval
-            elif mkwd == "simple_SFR":
-                if z <= 1:
-                    return (1 + z) ** mval
-                return (1 + 1) ** 3.6 * (1 + z) ** (mval - 3.6)
-        raise Exception("Unknown source evo type: {:}".format(m))
+        return evolution_factor(self.source_evo_m, z)
 
 
 class SimpleSource(CosmicRaySource):
@@ -126,7 +145,7 @@ class BrokenEvolutionSource(AugerFitSource):
 
     def evolution(self, z):
         if z < 0:
-            raise Exception("Source evolution not defined for negative z = {:}".format(z))
+            raise ValueError(f"source evolution is defined for z >= 0 only (z = {z})")
         if z < self.z_break:
             return (1 + z) ** self.m_break
         return (1 + self.z_break) ** self.m_break

@@ -1,37 +1,41 @@
 """Photo-nuclear rate kernel and continuous loss rates.
 
-``PhotoNuclearInteractionRate`` builds the dense FP64 fold kernel ``B``
-(``batch_dim x dph``) whose product with the photon density gives the values
-of the coupling matrix M on a fixed CSR pattern:
+Rate kernel.  With photon density n(eps, z) sampled at the photon-grid
+centres, the coupling matrix M of the nuclear states has fixed CSR pattern
+and values
 
-    M.data = B @ n_gamma(eps_ph, z)            (interaction_rates.py:966-1067)
+    M.data = B @ n(eps, z)                     (interaction_rates.py:966-1067)
 
-Construction follows the reference's log-grid / Toeplitz recipe
-(``interaction_rates.py:251-646``):
+where each row of the dense FP64 kernel ``B`` belongs to one (daughter state,
+mother state) entry.  Because the cosmic-ray centres, cosmic-ray edges and
+photon edges share one logarithmic step, the integral of a response function
+over a photon bin, seen from mother energy bin i, only depends on i + j
+(photon bin j) -- and for redistributed daughters additionally on the bin
+offset k - i of the daughter energy.  So per channel one table of
+antiderivative differences on a log grid gives every row by shifting a
+window (the reference's construction, ``interaction_rates.py:251-646``):
 
-* boost-conserving channel tile  B[i_mo, j] = (m_p/E_mo) dR[i_mo + j]
-  minus the non-elastic tile on the (mo, mo) diagonal;
-* redistributed channel tile
-  B[i_mo, i_da, j] = (m_p/E_mo)(dE_mo/dE_da) ddR[i_mo + j, i_da - i_mo + d - 1],
-  clipped at 0 before the non-elastic subtraction, kept where
-  x_cut <= bcr[i_da]/ecr[i_mo] <= 1 (x_cut_proton for proton daughters);
-* rows are lexsorted by (row, col) so that row k of ``B`` feeds entry k of
-  the canonical CSR ``coupling_mat.data`` (``interaction_rates.py:648-697``).
+* boost-conserving daughter (or a mother's own non-elastic loss): mother bin
+  i contributes the row (m_p / E_i) (dR_incl - dR_nonel)[i + j];
+* redistributed daughter bin k of mother bin i: the row
+  (m_p / E_i)(dE_i / dE_k) ddR[i + j, k - i + d - 1], negative values set to
+  zero, the mother's non-elastic row subtracted afterwards when k = i and
+  the daughter is the mother; only pairs with x_cut <= E_k,low / E_i <= 1
+  are kept (x_cut_proton for proton daughters).
 
-The 1-D antiderivatives use the same linear operator as the reference
-(a k=1 zero-extrapolating spline's antiderivative sampled on the y grid,
-``interaction_rates.py:25-47``); the 2-D ones use bilinear interpolation with
-zero fill (``util.py:106-160``).  Both are evaluated with scipy, the library
-the reference itself calls for them.
+The 1-D antiderivative of a response function is that of its piecewise-
+linear interpolant (zero outside the tabulated range), sampled on the y
+grid (``interaction_rates.py:25-47``); the 2-D double antiderivative is
+sampled bilinearly with zero fill (``util.py:106-160``).  Rows are ordered as
+the entries of the canonical CSR ``coupling_mat`` (``interaction_rates.py:
+648-697``); ``_batch_matrix``, ``_batch_rows``, ``_batch_cols`` and
+``coupling_mat`` are the reference's attribute names for them.
 
-Loss rates: adiabatic kappa = H(z) cm2sec E (``1194-1235``) and the
-Bethe-Heitler pair-production rate (Blumenthal phi(xi) fit, 400 log-spaced
-xi, ``1238-1386``).
+Loss rates: adiabatic kappa = H(z) cm2sec E (``interaction_rates.py:
+1194-1235``) and Bethe-Heitler pair production (``1238-1386``).
 """
 
 from __future__ import annotations
-
-import itertools
 
 import numpy as np
 from scipy.integrate import trapezoid
@@ -44,23 +48,61 @@ from .units import PDG_PROTON, PRINCE_UNITS, is_nucleus
 
 
 def response_integral_operator(ygr, y_grid):
-    """Matrix M with M @ f = antiderivative of the k=1 spline of f, on y_grid.
+    """Matrix Q with (Q @ f)[m] = integral of the piecewise-linear
+    interpolant of f (knots ygr, zero outside them) from ygr[0] to y_grid[m].
 
-    Zero outside [ygr[0], ygr[-1]] (ext='zeros'), as the reference's
-    ``_response_integral_operator`` (interaction_rates.py:25-47).
+    Column k is that antiderivative for the unit vector e_k (the operator the
+    reference builds from k = 1 splines, interaction_rates.py:25-47).
     """
     n = len(ygr)
-    M = np.empty((len(y_grid), n))
-    e = np.zeros(n)
+    Q = np.empty((len(y_grid), n))
+    unit = np.zeros(n)
     for k in range(n):
-        e[k] = 1.0
-        M[:, k] = InterpolatedUnivariateSpline(ygr, e, k=1, ext="zeros").antiderivative()(y_grid)
-        e[k] = 0.0
-    return M
+        unit[k] = 1.0
+        spline = InterpolatedUnivariateSpline(ygr, unit, k=1, ext="zeros")
+        Q[:, k] = spline.antiderivative()(y_grid)
+        unit[k] = 0.0
+    return Q
+
+
+class _ResponseDifferences:
+    """Per-channel differences of the response antiderivatives on the log grids.
+
+    ``nonel[mo]`` / ``incl[(mo, da)]``: length d + dph - 1 vectors, entry
+    m = R(y[m + 1]) - R(y[m]).  ``redist[(mo, da)]``: the second difference
+    of the double antiderivative, shape (d + dph - 1, 2 d - 1), over
+    (y cells, x cells).
+    """
+
+    def __init__(self, resp, y_grid, x_grid):
+        Q = response_integral_operator(resp.response_ygrid, y_grid)
+        self.nonel = self._first_differences(resp.nonel_keys, resp.nonel_f_stack, Q)
+        self.incl = self._first_differences(resp.incl_keys, resp.incl_f_stack, Q)
+        self.redist = {}
+        keys = list(resp.incl_diff_integral_keys)
+        if keys:
+            sampler = RegularGridInterpolator(
+                (resp.xcenters, resp.response_ygrid), resp.incl_diff_integral_stack,
+                method="linear", bounds_error=False, fill_value=0.0,
+            )
+            yy, xx = np.meshgrid(y_grid, x_grid, indexing="ij")
+            pts = np.column_stack([xx.ravel(), yy.ravel()])
+            R2 = sampler(pts).reshape(y_grid.size, x_grid.size, len(keys))
+            # corner sums of every (y cell, x cell): the integral over the cell
+            cell = (R2[1:, 1:, :] - R2[1:, :-1, :]) - R2[:-1, 1:, :] + R2[:-1, :-1, :]
+            for c, key in enumerate(keys):
+                self.redist[key] = np.ascontiguousarray(cell[:, :, c])
+
+    @staticmethod
+    def _first_differences(keys, fstack, Q):
+        if not len(keys):
+            return {}
+        diffs = np.diff(fstack @ Q.T, axis=1)
+        return {key: diffs[row] for row, key in enumerate(keys)}
 
 
 class PhotoNuclearInteractionRate:
-    """Dense FP64 fold kernel + fixed-pattern coupling matrix."""
+    """Dense FP64 rate kernel ``_batch_matrix`` + fixed-pattern ``coupling_mat``."""
 
     def __init__(self, prince_run):
         self.prince_run = prince_run
@@ -69,8 +111,9 @@ class PhotoNuclearInteractionRate:
         self.e_photon = prince_run.ph_grid
         self.e_cosmicray = prince_run.cr_grid
         self._ratemat_zcache = None
-        self._init_matrices()
-        self._init_coupling_mat()
+        self._check_common_log_step()
+        rows, cols, vals = self._assemble_channels()
+        self._store_in_csr_order(rows, cols, vals)
 
     @property
     def photon_field(self):
@@ -78,154 +121,132 @@ class PhotoNuclearInteractionRate:
 
     def photon_vector(self, z, pfield=None):
         """Photon density at the photon-grid centres (interaction_rates.py:105-116)."""
-        pf = pfield if pfield is not None else self.photon_field
-        return pf.get_photon_density(self.e_photon.grid, z)
+        field = self.photon_field if pfield is None else pfield
+        return field.get_photon_density(self.e_photon.grid, z)
 
-    # ------------------------------------------------------------------
-    def _assert_log_grids_compatible(self):
-        ecr, bcr, bph = self.e_cosmicray.grid, self.e_cosmicray.bins, self.e_photon.bins
-        s_ecr = np.log10(ecr[1] / ecr[0])
-        s_bcr = np.log10(bcr[1] / bcr[0])
-        s_bph = np.log10(bph[1] / bph[0])
-        if not (np.isclose(s_ecr, s_bcr) and np.isclose(s_ecr, s_bph)):
+    # ------------------------------------------------------------ construction
+    def _check_common_log_step(self):
+        """The shifted-window construction needs one log step on all three grids."""
+        steps = [np.log10(g[1] / g[0]) for g in
+                 (self.e_cosmicray.grid, self.e_cosmicray.bins, self.e_photon.bins)]
+        if not (np.isclose(steps[0], steps[1]) and np.isclose(steps[0], steps[2])):
             raise RuntimeError(
-                "Kernel construction requires the cosmic-ray and photon grids to share "
-                f"the same log-step (got {s_ecr:.6f}, {s_bcr:.6f}, {s_bph:.6f})."
+                "rate kernel: cosmic-ray centres, cosmic-ray edges and photon edges must share "
+                "one log step (got %.6f, %.6f, %.6f)" % tuple(steps)
             )
 
-    def _init_matrices(self):
-        self._assert_log_grids_compatible()
-        sp_id_ref = self.spec_man.pdgid2sref
-        resp = self.cross_sections.resp
-        m_pr = PRINCE_UNITS.m_proton
-        ecr = self.e_cosmicray.grid
-        bcr = self.e_cosmicray.bins
-        bph = self.e_photon.bins
-        dcr = self.e_cosmicray.d
-        dph = self.e_photon.d
-        delta_ec = self.e_cosmicray.widths
+    def _log_grids(self):
+        """y grid (d + dph points) and x grid (2 d points) of the antiderivative tables."""
+        ecr, bcr, bph = self.e_cosmicray.grid, self.e_cosmicray.bins, self.e_photon.bins
+        d, dph = self.e_cosmicray.d, self.e_photon.d
+        step = np.log10(ecr[1] / ecr[0])
+        y_grid = (ecr[0] * bph[0] / PRINCE_UNITS.m_proton) * 10 ** (np.arange(d + dph) * step)
+        x_grid = (bcr[0] / ecr[0]) * 10 ** ((np.arange(2 * d) - (d - 1)) * step)
+        return y_grid, x_grid
 
-        log_step = np.log10(ecr[1] / ecr[0])
-        y0 = ecr[0] * bph[0] / m_pr
-        y_grid = y0 * 10 ** (np.arange(dcr + dph) * log_step)
-        x0 = bcr[0] / ecr[0]
-        x_grid = x0 * 10 ** ((np.arange(2 * dcr) - (dcr - 1)) * log_step)
-        self._y_grid, self._x_grid = y_grid, x_grid
-
-        factor_mo = m_pr / ecr
-        factor_diff = factor_mo[:, None] * (delta_ec[:, None] / delta_ec[None, :])
-        i_idx = np.arange(dcr)
-        j_idx = np.arange(dph)
-        A_2d = i_idx[:, None] + j_idx[None, :]
-        B_idx = (i_idx[None, :] - i_idx[:, None]) + (dcr - 1)
-
-        Mop = response_integral_operator(resp.response_ygrid, y_grid)
-        nonel_dR, incl_dR = {}, {}
-        if resp.nonel_f_stack.size:
-            dRn = np.diff(resp.nonel_f_stack @ Mop.T, axis=1)
-            nonel_dR = {mo: dRn[i] for i, mo in enumerate(resp.nonel_keys)}
-        if resp.incl_f_stack.size:
-            dRi = np.diff(resp.incl_f_stack @ Mop.T, axis=1)
-            incl_dR = {key: dRi[i] for i, key in enumerate(resp.incl_keys)}
-
-        diff_ddR = {}
-        if resp.incl_diff_integral_keys:
-            Y2d, X2d = np.meshgrid(y_grid, x_grid, indexing="ij")
-            rgi = RegularGridInterpolator(
-                (resp.xcenters, resp.response_ygrid),
-                resp.incl_diff_integral_stack,
-                method="linear",
-                bounds_error=False,
-                fill_value=0.0,
-            )
-            R_stack = rgi(np.column_stack([X2d.ravel(), Y2d.ravel()])).reshape(
-                *Y2d.shape, len(resp.incl_diff_integral_keys)
-            )
-            ddR_stack = (
-                R_stack[1:, 1:, :] - R_stack[1:, :-1, :] - R_stack[:-1, 1:, :] + R_stack[:-1, :-1, :]
-            )
-            for i, key in enumerate(resp.incl_diff_integral_keys):
-                diff_ddR[key] = np.ascontiguousarray(ddR_stack[..., i])
-
-        xl_2d = bcr[:-1][None, :] / ecr[:, None]
-        diag = np.arange(dcr)
-        imo_grid, ida_grid = np.meshgrid(i_idx, i_idx, indexing="ij")
-        bc_set = self.cross_sections.known_bc_channels_set
-        diff_set = self.cross_sections.known_diff_channels_set
-
-        tiles, rows_l, cols_l = [], [], []
-        known_species_rev = self.spec_man.known_species[::-1]
-        for moid, daid in itertools.product(known_species_rev, known_species_rev):
-            if not is_nucleus(moid):
+    def _channels(self):
+        """(mother, daughter, kind) of every channel of the run; kind is
+        "bc" (boost-conserving or the mother's own non-elastic loss) or
+        "redist".  Mothers are nuclei; both ends must be transported species."""
+        cs = self.cross_sections
+        species = set(self.spec_man.known_species)
+        bc = cs.known_bc_channels_set
+        redist = cs.known_diff_channels_set
+        nonel_mothers = set(cs.resp.nonel_keys)
+        pairs = set(bc) | set(redist) | {(m, m) for m in nonel_mothers}
+        out = []
+        for mo, da in sorted(pairs):
+            if mo not in species or da not in species or not is_nucleus(mo):
                 continue
-            has_nonel = (moid == daid) and (moid in nonel_dR)
-            in_bc = (moid, daid) in bc_set
-            in_diff = (moid, daid) in diff_set
-            if in_bc or (has_nonel and not in_diff):
-                has_incl = (moid, daid) in incl_dR
-                if not (has_nonel or has_incl):
-                    raise Exception("Channel without interactions:", (moid, daid))
-                tile = np.zeros((dcr, dph))
-                if has_incl:
-                    tile += factor_mo[:, None] * incl_dR[(moid, daid)][A_2d]
-                if has_nonel:
-                    tile -= factor_mo[:, None] * nonel_dR[moid][A_2d]
-                tiles.append(tile)
-                rows_l.append(sp_id_ref[daid].lidx() + i_idx)
-                cols_l.append(sp_id_ref[moid].lidx() + i_idx)
-            elif in_diff:
-                ddR = diff_ddR[(moid, daid)]
-                res = factor_diff[:, :, None] * ddR[A_2d[:, None, :], B_idx[:, :, None]]
-                np.clip(res, 0.0, None, out=res)
-                if has_nonel:
-                    res[diag, diag, :] -= factor_mo[:, None] * nonel_dR[moid][A_2d]
-                cut_low = config.x_cut_proton if daid == PDG_PROTON else config.x_cut
-                cuts2d = np.logical_and(xl_2d >= cut_low, xl_2d <= 1)
-                kept = res[cuts2d]
-                if kept.shape[0] == 0:
-                    continue
-                tiles.append(kept)
-                rows_l.append(sp_id_ref[daid].lidx() + ida_grid[cuts2d])
-                cols_l.append(sp_id_ref[moid].lidx() + imo_grid[cuts2d])
+            own_loss = mo == da and mo in nonel_mothers
+            if (mo, da) in bc or (own_loss and (mo, da) not in redist):
+                out.append((mo, da, "bc"))
+            else:
+                out.append((mo, da, "redist"))
+        return out
 
-        self._batch_matrix = np.concatenate(tiles, axis=0)
-        self._batch_rows = np.concatenate(rows_l).astype(np.int64)
-        self._batch_cols = np.concatenate(cols_l).astype(np.int64)
+    def _assemble_channels(self):
+        d, dph = self.e_cosmicray.d, self.e_photon.d
+        ecr, bcr = self.e_cosmicray.grid, self.e_cosmicray.bins
+        width = self.e_cosmicray.widths
+        y_grid, x_grid = self._log_grids()
+        tabs = _ResponseDifferences(self.cross_sections.resp, y_grid, x_grid)
 
-    def _init_coupling_mat(self):
+        ibin = np.arange(d)
+        window = ibin[:, None] + np.arange(dph)[None, :]  # [mother bin, photon bin]
+        mother_scale = PRINCE_UNITS.m_proton / ecr  # per mother bin
+        # redistributed rows: [mother bin i, daughter bin k]
+        redist_scale = mother_scale[:, None] * (width[:, None] / width[None, :])
+        x_col = (ibin[None, :] - ibin[:, None]) + (d - 1)
+        x_low = bcr[:-1][None, :] / ecr[:, None]
+        mo_bin, da_bin = np.meshgrid(ibin, ibin, indexing="ij")
+
+        sref = self.spec_man.pdgid2sref
+        rows, cols, vals = [], [], []
+        for mo, da, kind in self._channels():
+            r0, c0 = sref[da].lidx(), sref[mo].lidx()
+            loss = tabs.nonel.get(mo) if mo == da else None
+            if kind == "bc":
+                gain = tabs.incl.get((mo, da))
+                if gain is None and loss is None:
+                    raise ValueError(f"rate kernel: channel {mo} -> {da} has no cross section")
+                tile = mother_scale[:, None] * gain[window] if gain is not None else 0.0
+                if loss is not None:
+                    tile = tile - mother_scale[:, None] * loss[window]
+                vals.append(np.broadcast_to(tile, (d, dph)).copy())
+                rows.append(r0 + ibin)
+                cols.append(c0 + ibin)
+                continue
+            cube = redist_scale[:, :, None] * tabs.redist[(mo, da)][window[:, None, :], x_col[:, :, None]]
+            np.maximum(cube, 0.0, out=cube)
+            if loss is not None:
+                cube[ibin, ibin, :] -= mother_scale[:, None] * loss[window]
+            x_min = config.x_cut_proton if da == PDG_PROTON else config.x_cut
+            keep = (x_low >= x_min) & (x_low <= 1)
+            if not keep.any():
+                continue
+            vals.append(cube[keep])
+            rows.append(r0 + da_bin[keep])
+            cols.append(c0 + mo_bin[keep])
+        return (np.concatenate(rows).astype(np.int64), np.concatenate(cols).astype(np.int64),
+                np.concatenate(vals, axis=0))
+
+    def _store_in_csr_order(self, rows, cols, vals):
+        """Sort the kernel rows into CSR order of (row, col) and build the pattern."""
         n = self.prince_run.dim_states
-        nrows = self._batch_matrix.shape[0]
-        self.coupling_mat = csr_matrix(
-            (np.zeros(nrows), (self._batch_rows, self._batch_cols)), shape=(n, n), copy=True
-        )
-        if self.coupling_mat.nnz != nrows:
-            raise RuntimeError("duplicate (row, col) entries in the rate kernel")
-        self.sortidx = np.lexsort((self._batch_cols, self._batch_rows))
-        self._batch_rows = self._batch_rows[self.sortidx]
-        self._batch_cols = self._batch_cols[self.sortidx]
-        self._batch_matrix = np.ascontiguousarray(self._batch_matrix[self.sortidx, :])
-        self._batch_vec = np.zeros(nrows)
+        order = np.lexsort((cols, rows))
+        rows, cols = rows[order], cols[order]
+        if rows.size > 1 and np.any((np.diff(rows) == 0) & (np.diff(cols) == 0)):
+            raise RuntimeError("rate kernel: two channels write the same matrix entry")
+        indptr = np.searchsorted(rows, np.arange(n + 1), side="left")
+        self.coupling_mat = csr_matrix((np.zeros(rows.size), cols, indptr), shape=(n, n))
+        self.sortidx = order
+        self._batch_rows = rows
+        self._batch_cols = cols
+        self._batch_matrix = np.ascontiguousarray(vals[order, :])
+        self._batch_vec = np.zeros(rows.size)
 
-    # ------------------------------------------------------------------
-    @property
-    def batch_matrix(self):
-        """Dense FP64 fold kernel, rows in canonical CSR order of ``coupling_mat``."""
-        return self._batch_matrix
-
-    def fold(self, photon):
-        """M.data values for a given photon vector (one dense GEMV)."""
-        return self._batch_matrix @ np.asarray(photon, dtype=np.float64)
-
+    # ------------------------------------------------------------ evaluation
     def get_hadr_jacobian(self, z, scale_fac=1.0, force_update=False, pfield=None):
-        """Coupling matrix with values folded at redshift z (1066-1120)."""
+        """Coupling matrix with values folded at redshift z (interaction_rates.py:1066-1120)."""
         if force_update or pfield is not None or self._ratemat_zcache != z:
             np.dot(self._batch_matrix, self.photon_vector(z, pfield=pfield), out=self._batch_vec)
             self._ratemat_zcache = None if pfield is not None else z
             if scale_fac == 1.0:
-                np.copyto(self.coupling_mat.data, self._batch_vec)
+                self.coupling_mat.data[:] = self._batch_vec
             else:
                 np.multiply(self._batch_vec, scale_fac, out=self.coupling_mat.data)
         return self.coupling_mat
+
+
+def _state_energies(prince_run, energy):
+    """(energy per state, slice bounds per species) on grid centres or bin edges."""
+    sm, grid = prince_run.spec_man, prince_run.cr_grid
+    if energy == "grid":
+        return grid.grid, prince_run.dim_states, (lambda s: (s.lidx(), s.uidx()))
+    if energy == "bins":
+        return grid.bins, prince_run.dim_bins, (lambda s: (s.lbin(), s.ubin()))
+    raise ValueError(f"energy must be 'grid' or 'bins', got {energy!r}")
 
 
 class ContinuousAdiabaticLossRate:
@@ -234,28 +255,56 @@ class ContinuousAdiabaticLossRate:
     def __init__(self, prince_run, energy="grid"):
         self.spec_man = prince_run.spec_man
         self.e_cosmicray = prince_run.cr_grid
-        if energy == "grid":
-            earr, dim = self.e_cosmicray.grid, prince_run.dim_states
-            lo, hi = (lambda s: s.lidx()), (lambda s: s.uidx())
-        else:
-            earr, dim = self.e_cosmicray.bins, prince_run.dim_bins
-            lo, hi = (lambda s: s.lbin()), (lambda s: s.ubin())
+        earr, dim, bounds = _state_energies(prince_run, energy)
         self.energy_vector = np.zeros(dim)
         for spec in self.spec_man.species_refs:
-            self.energy_vector[lo(spec) : hi(spec)] = earr
+            lo, hi = bounds(spec)
+            self.energy_vector[lo:hi] = earr
 
     def loss_vector(self, z, energy=None):
-        if energy is None:
-            return H(z) * PRINCE_UNITS.cm2sec * self.energy_vector
-        return H(z) * PRINCE_UNITS.cm2sec * energy
+        rate = H(z) * PRINCE_UNITS.cm2sec
+        return rate * (self.energy_vector if energy is None else energy)
+
+
+#: coefficients of phi(xi) (interaction_rates.py:1346-1386): the rational fit
+#: below xi = 25 in u = xi - 2, Blumenthal's cubic in ln xi and its 1/xi series
+PHI_LOW = (0.8048, 0.1459, 1.137e-3, -3.879e-6)
+PHI_HIGH = (2.667, -14.45, 50.95, -86.07)
+PHI_HIGH_SERIES = (2.91, 78.35, 1837)
+
+
+def blumenthal_phi(xi, low=PHI_LOW, high=PHI_HIGH, series=PHI_HIGH_SERIES):
+    """phi(xi) of Bethe-Heitler energy loss (Blumenthal 1970), as the reference
+    evaluates it: below xi = 25 (pi/12) u^4 / S(u) with S(u) = c1 u + c2 u^2 +
+    c3 u^3 + c4 u^4, u = xi - 2 (the Chodorowski, Zdziarski & Sikora 1992 fit
+    coefficients; their published denominator is 1 + S(u), see the tests),
+    above it xi P(ln xi) / (1 - f1/xi - f2/xi^2 - f3/xi^3) with Blumenthal's
+    ultra-relativistic cubic P."""
+    xi = np.asarray(xi, dtype=np.float64)
+    c1, c2, c3, c4 = low
+    a3, a2, a1, a0 = high
+    f1, f2, f3 = series
+    out = np.zeros(xi.shape)
+    below = xi < 25.0
+    u = xi[below] - 2
+    S = c1 * u**1 + c2 * u**2 + c3 * u**3 + c4 * u**4
+    out[below] = np.pi / 12.0 * u**4 / S
+    v = xi[~below]
+    t = np.log(v)
+    P = ((a3 * t + a2) * t + a1) * t + a0
+    out[~below] = (v * P) / (1 - f1 * v**-1 - f2 * v**-2 - f3 * v**-3)
+    return out
 
 
 class ContinuousPairProductionLossRate:
-    """Bethe-Heitler pair-production loss rate (interaction_rates.py:1238-1386).
+    """Bethe-Heitler pair-production loss rate of nuclei (interaction_rates.py:1238-1386).
 
-    kappa_pair(z) = scale_vec * rate(E, z), with
-    rate = trapz(n(m_e xi/(2 gamma), z) phi(xi)/xi^2, xi) and
-    scale_vec = alpha r_e^2 m_e^2 Z^2/A on nucleus blocks.
+    loss_vector(z) = scale_vec * r(E, z) on every nucleus block with
+    r = trapz(n(m_e xi / (2 gamma), z) phi(xi) / xi^2, xi) over ``xi_steps``
+    log-spaced xi in [2 + 1e-8, 1e16] (gamma = E / m_p per nucleon) and
+    scale_vec = alpha r_e^2 m_e^2 Z^2 / A.  ``xi``, ``phi_xi2``,
+    ``photon_grid``, ``photon_vector`` and ``scale_vec`` carry the
+    reference's names.
     """
 
     def __init__(self, prince_run, energy="grid", xi_steps=400):
@@ -263,61 +312,39 @@ class ContinuousPairProductionLossRate:
         self.spec_man = prince_run.spec_man
         self.e_cosmicray = prince_run.cr_grid
         self.xi = np.logspace(np.log10(2 + 1e-8), 16.0, xi_steps)
-        self.phi_xi2 = self._phi(self.xi) / (self.xi**2)
-        if energy == "grid":
-            earr, dim = self.e_cosmicray.grid, prince_run.dim_states
-            self._lo, self._hi = (lambda s: s.lidx()), (lambda s: s.uidx())
-        else:
-            earr, dim = self.e_cosmicray.bins, prince_run.dim_bins
-            self._lo, self._hi = (lambda s: s.lbin()), (lambda s: s.ubin())
+        self.phi_xi2 = blumenthal_phi(self.xi) / (self.xi**2)
+        earr, dim, bounds = _state_energies(prince_run, energy)
+        self._bounds = bounds
         self._loss_dim = dim
         units = PRINCE_UNITS.fine_structure * PRINCE_UNITS.r_electron**2 * PRINCE_UNITS.m_electron**2
         self.scale_vec = np.zeros(dim)
-        self.species_scale = np.zeros(self.spec_man.nspec)
         for spec in self.spec_man.species_refs:
-            if not spec.is_nucleus:
-                continue
-            val = units * abs(spec.charge) ** 2 / float(spec.A)
-            self.scale_vec[self._lo(spec) : self._hi(spec)] = val * np.ones_like(earr)
-            self.species_scale[spec.princeidx] = val
+            if spec.is_nucleus:
+                lo, hi = bounds(spec)
+                self.scale_vec[lo:hi] = units * abs(spec.charge) ** 2 / float(spec.A)
+        # target-photon energy of every (nucleon energy, xi) sample
         gamma = earr / PRINCE_UNITS.m_proton
         self.photon_grid = np.outer(1 / gamma, self.xi) * PRINCE_UNITS.m_electron / 2.0
-        self.pg_desort = self.photon_grid.reshape(-1).argsort()
-        self.pg_sorted = self.photon_grid.reshape(-1)[self.pg_desort]
+        flat = self.photon_grid.reshape(-1)
+        self._ascending = flat.argsort()
+        self._ascending_energies = flat[self._ascending]
 
     @property
     def photon_field(self):
         return self.prince_run.photon_field
 
     def photon_vector(self, z, pfield=None):
-        pf = pfield if pfield is not None else self.photon_field
-        pv = np.zeros_like(self.photon_grid)
-        pv.reshape(-1)[self.pg_desort] = pf.get_photon_density(self.pg_sorted, z)
-        return pv
-
-    def rate_single(self, z, pfield=None):
-        """Per-nucleon-energy rate shared by every nucleus block (length d)."""
-        return trapezoid(self.photon_vector(z, pfield=pfield) * self.phi_xi2, self.xi, axis=1)
+        """n(photon_grid, z): the field is evaluated once on the ascending energies."""
+        field = self.photon_field if pfield is None else pfield
+        out = np.zeros_like(self.photon_grid)
+        out.reshape(-1)[self._ascending] = field.get_photon_density(self._ascending_energies, z)
+        return out
 
     def loss_vector(self, z, pfield=None):
-        rs = self.rate_single(z, pfield=pfield)
-        rate_full = np.zeros(self._loss_dim)
+        per_energy = trapezoid(self.photon_vector(z, pfield=pfield) * self.phi_xi2, self.xi, axis=1)
+        full = np.zeros(self._loss_dim)
         for spec in self.spec_man.species_refs:
             if spec.is_nucleus:
-                rate_full[self._lo(spec) : self._hi(spec)] = rs
-        return self.scale_vec * rate_full
-
-    @staticmethod
-    def _phi(xi):
-        """Blumenthal (1970) phi(xi) with the Chodorowski et al. fit below 25."""
-        bltal_ultrarel = np.poly1d([2.667, -14.45, 50.95, -86.07])
-        c1, c2, c3, c4 = 0.8048, 0.1459, 1.137e-3, -3.879e-6
-        f1, f2, f3 = 2.91, 78.35, 1837
-        res = np.zeros(xi.shape)
-        le = np.where(xi < 25.0)
-        he = np.where(xi >= 25.0)
-        xl = xi[le] - 2
-        res[le] = np.pi / 12.0 * xl**4 / (c1 * xl**1 + c2 * xl**2 + c3 * xl**3 + c4 * xl**4)
-        xh = xi[he]
-        res[he] = (xh * bltal_ultrarel(np.log(xh))) / (1 - f1 * xh**-1 - f2 * xh**-2 - f3 * xh**-3)
-        return res
+                lo, hi = self._bounds(spec)
+                full[lo:hi] = per_energy
+        return self.scale_vec * full

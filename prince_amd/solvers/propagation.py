@@ -44,6 +44,14 @@ from .layout import (
     transposed_layout,
 )
 from .result import UHECRPropagationResult
+from .run_inputs import (
+    em_cascade_grid,
+    em_slices,
+    fd_block,
+    fold_kernel_fp64,
+    pair_rate_per_energy,
+    pair_species_scale,
+)
 
 
 def z_grid_for(initial_z, final_z, dz):
@@ -134,8 +142,9 @@ class UHECRPropagationSolver:
         self.dim_states = prince_run.dim_states
         # co-evolved EM cascade (propagation.py:250-316): photon and e+- blocks
         # follow the nuclear states in ``state``
-        self.enable_em_cascade = bool(getattr(prince_run, "enable_em_cascade", False))
-        self.dim_em = prince_run.dim_em if self.enable_em_cascade else 0
+        em_grid = em_cascade_grid(prince_run)
+        self.enable_em_cascade = em_grid is not None
+        self.dim_em = 3 * em_grid.d if em_grid is not None else 0
         self._em_dz = 1e-2
         self.state = np.zeros(prince_run.dim_states + self.dim_em)
         self.result = None
@@ -149,7 +158,7 @@ class UHECRPropagationSolver:
     def _em_info(self):
         """{pdg: (grid, slice)} of the EM blocks of ``state`` (empty without EM)."""
         run = self.prince_run
-        return {p: (run.em_grid.grid, sl) for p, sl in run.em_slices().items()}
+        return {p: (run.em_grid.grid, sl) for p, sl in em_slices(run).items()}
 
     @property
     def res(self):
@@ -217,7 +226,7 @@ class _HipDeviceCache:
 
     def __init__(self, run, device):
         torch = _torch()
-        B = np.ascontiguousarray(run.int_rates.batch_matrix, dtype=np.float64)
+        B = fold_kernel_fp64(run.int_rates)
         self.B_full = torch.from_numpy(B).to(device)
         self.nrows, self.dph = B.shape
         self.layouts = {}
@@ -336,7 +345,9 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
 
     def _ensure_D_split(self):
         if self._D_tables is None:
-            dcoef, ddiag = fd_tables(self.diff_operator.block)
+            self._fd_block = fd_block(self.diff_operator.operator, self.spec_man.nspec,
+                                      self.prince_run.cr_grid.d)
+            dcoef, ddiag = fd_tables(self._fd_block)
             self._D_tables = (dcoef, ddiag)
 
     def _init_state(self):
@@ -482,7 +493,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         if split is None:
             phot = np.stack([ir.photon_vector(z) for z in zs]).astype(np.float64)
             if self.enable_pairprod_losses:
-                rpair = np.stack([pair.rate_single(z) for z in zs])
+                rpair = np.stack([pair_rate_per_energy(pair, z) for z in zs])
             else:
                 rpair = np.zeros((len(zs), d))
             return phot, rpair, None, None
@@ -497,8 +508,8 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         phot = np.stack([ir.photon_vector(z, pfield=base) for z in zs]).astype(np.float64)
         phot_e = np.stack([ir.photon_vector(z, pfield=comp) for z in zs]).astype(np.float64)
         if self.enable_pairprod_losses:
-            rpair = np.stack([pair.rate_single(z, pfield=base) for z in zs])
-            rpair_e = np.stack([pair.rate_single(z, pfield=comp) for z in zs])
+            rpair = np.stack([pair_rate_per_energy(pair, z, pfield=base) for z in zs])
+            rpair_e = np.stack([pair_rate_per_energy(pair, z, pfield=comp) for z in zs])
         else:
             rpair = np.zeros((len(zs), d))
             rpair_e = np.zeros((len(zs), d))
@@ -602,7 +613,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         if wide:
             src_grid = src_grid.transpose(0, 1, 3, 2)  # [slot][J][d][K]
         dcoef, ddiag = self._D_tables
-        pscale = np.asarray(self.pair_loss_rates_grid.species_scale, dtype=np.float64)
+        pscale = pair_species_scale(self.pair_loss_rates_grid, self.spec_man)
 
         def dv(a, dtype=None):
             a = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype=dtype))
@@ -699,7 +710,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             tasks_t = build_tasks(wl.tr, self.part_cost_for(K), _lib.WG_WAVES)
             bufs["tasks_t"] = dv(tasks_t, np.int32)
             bufs["blocks_t"] = dv(wl.blocks_t if wl.blocks_t.size else np.zeros((1, 4)), np.int32)
-            bufs["dcoef_t"] = dv(fd_transposed_tables(self.diff_operator.block), np.float64)
+            bufs["dcoef_t"] = dv(fd_transposed_tables(self._fd_block), np.float64)
             opT = make_op(bufs["tasks_t"], bufs["blocks_t"], bufs["dcoef_t"])
         sch = _lib.PrhSchedule(
             nsteps=len(h), pad_=0, h=bufs["h"].data_ptr(), dldz=bufs["dldz"].data_ptr(),
@@ -759,13 +770,15 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         dE = np.gradient(E_p)
         pair = self.pair_loss_rates_grid
         bhw = np.stack([
-            (PRINCE_UNITS.c * self._em_dz / ((1.0 + z) * H(z))) * (dE / E_p) * pair.rate_single(z)
+            (PRINCE_UNITS.c * self._em_dz / ((1.0 + z) * H(z))) * (dE / E_p)
+            * pair_rate_per_energy(pair, z)
             for z in zs
         ])
+        pscale = pair_species_scale(pair, self.spec_man)
         cbh = np.zeros(self.spec_man.nspec)
         for sref in self.spec_man.species_refs:
             if sref.is_nucleus and abs(sref.charge) >= 1:
-                cbh[sref.princeidx] = float(sref.A) * pair.species_scale[sref.princeidx]
+                cbh[sref.princeidx] = float(sref.A) * pscale[sref.princeidx]
         dcoef, ddiag = fd_tables(single_block(eg))
         dev = self.device
         gam0 = np.zeros((eg.d, K))
@@ -819,12 +832,18 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         plan = plan if plan is not None else self._plan
         if k_begin == 0:
             plan["x"].zero_()
-            if plan.get("em_bufs") is not None:
-                plan["em_bufs"]["gam"].zero_()
+            self._reset_em_state(plan)
             self.fold_windows(plan)
         k_end = plan["nsteps"] if k_end is None else k_end
         self._launch_steps(plan, k_begin, k_end)
         return plan["x"]
+
+    @staticmethod
+    def _reset_em_state(plan):
+        """Photon state back to the solve's initial EM state (step 0 reads ``gam``)."""
+        eb = plan.get("em_bufs")
+        if eb is not None:
+            eb["gam"].copy_(eb["gam0"])
 
     # ---------------------------------------------------------------- autotuner
     def tuning_candidates(self, K):
@@ -922,6 +941,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             plan["ckpt"] = ck
         plan["ckpt_chunk"] = chunk
         plan["x"].zero_()
+        self._reset_em_state(plan)
         self.fold_windows(plan)
         for c in range(nchk):
             ck[c].copy_(plan["x"])

@@ -30,6 +30,7 @@ from ..model.cosmology import H
 from ..model.photonfields import PLANCK_PREF, CMBPhotonSpectrum, CombinedPhotonField, SyntheticEBL
 from ..model.units import PRINCE_UNITS
 from .layout import build_tasks
+from .run_inputs import pair_species_scale
 from .propagation import (
     MultiRHSPropagationSolverETD2,
     _MultiRHSETD2SolverHIP,
@@ -210,7 +211,7 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
             kt //= 2
         src_slot, src_grid, slots = self._source_tables(col_sources, J)
         dcoef, ddiag = self._D_tables
-        pscale = np.asarray(self.pair_loss_rates_grid.species_scale, dtype=np.float64)
+        pscale = pair_species_scale(self.pair_loss_rates_grid, self.spec_man)
         has_src = any(len(c) for c in col_sources) and self.enable_injection_jacobian
 
         def dv(a, dtype=None):

@@ -1,5 +1,5 @@
 """Write the golden fixtures of this directory (run from the repository root:
-``python tests/golden/make_golden.py``).
+``python tests/golden/make_golden.py [file names]``; no names writes all).
 
 The reference cannot run here (its import fetches a database), so these are
 regression fixtures of the FP64 CPU oracle, whose functions are pinned by
@@ -12,7 +12,12 @@ case and the expected final state:
 * ``etd2_nuclear_si28_z0.1.npz``: the A <= 56 run (CMB + EBL seed 0), pure Si-28
   source with the broken (1+z)^3 evolution, z 0.1 -> 0;
 * ``etd2_proton_em_z0.05.npz``: protons on the CMB with the EM cascade, z 0.05 -> 0
-  (nuclear state and photon spectrum).
+  (nuclear state and photon spectrum);
+* ``etd2_nuclear_si28_z3.npz``: the A <= 56 plain solve at full length, pure
+  Si-28, z 3 -> 0 (3000 steps, 288 rate windows);
+* ``etd2_nuclear_batch512_z3_cols.npz``: columns 0, 1, 2, 3 and 511 of the
+  512 seeded source draws (seed 0) on the A <= 56 run, z 3 -> 0, with the
+  draws (gamma, log10 R_max, fractions, m) that define them.
 """
 
 import os
@@ -34,6 +39,27 @@ def case(name, run, src, zi, **extra):
     o = OracleETD2(zi, 0.0, run)
     o.add_source_class(src)
     o.solve(1e-3)
+    save(name, run, o, zi, **extra)
+
+
+#: columns of the K = 512 draws kept in the batch fixture
+BATCH_COLUMNS = (0, 1, 2, 3, 511)
+
+
+def batch_case(name, run, zi, K=512, seed=0, columns=BATCH_COLUMNS):
+    """Oracle solve of the chosen columns of ``batch_sources(run, K, seed)``."""
+    srcs = workloads.batch_sources(run, K, seed=seed)
+    o = OracleETD2(zi, 0.0, run, K=len(columns))
+    for i, k in enumerate(columns):
+        o.add_source_class(srcs[k], i)
+    o.solve(1e-3)
+    g, lr, fr, m = workloads.batch_draws(K, seed)
+    cols = np.asarray(columns)
+    save(name, run, o, zi, K=K, seed=seed, columns=cols, gamma=g[cols], lrmax=lr[cols],
+         frac=fr[cols], m=m[cols])
+
+
+def save(name, run, o, zi, **extra):
     out = {"initial_z": zi, "final_z": 0.0, "dz": 1e-3, "state": o.state,
            "egrid": run.cr_grid.grid, "pdg": np.array(run.spec_man.known_species),
            "refresh_zs": np.array(o.refresh_zs)}
@@ -45,10 +71,23 @@ def case(name, run, src, zi, **extra):
     print(name, o.state.shape, len(o.refresh_zs))
 
 
-if __name__ == "__main__":
+def main(only=()):
+    """Write every fixture, or only those whose file names are listed."""
+    want = (lambda name: not only or name in only)
     run = workloads.proton_run()
-    case("etd2_proton_cmb_z0.05.npz", run, workloads.proton_source(run), 0.05)
+    if want("etd2_proton_cmb_z0.05.npz"):
+        case("etd2_proton_cmb_z0.05.npz", run, workloads.proton_source(run), 0.05)
     run = workloads.nuclear_run()
-    case("etd2_nuclear_si28_z0.1.npz", run, workloads.silicon_source(run), 0.1)
-    run = workloads.proton_em_run()
-    case("etd2_proton_em_z0.05.npz", run, workloads.proton_source(run), 0.05)
+    if want("etd2_nuclear_si28_z0.1.npz"):
+        case("etd2_nuclear_si28_z0.1.npz", run, workloads.silicon_source(run), 0.1)
+    if want("etd2_nuclear_si28_z3.npz"):
+        case("etd2_nuclear_si28_z3.npz", run, workloads.silicon_source(run), 3.0)
+    if want("etd2_nuclear_batch512_z3_cols.npz"):
+        batch_case("etd2_nuclear_batch512_z3_cols.npz", run, 3.0)
+    if want("etd2_proton_em_z0.05.npz"):
+        run = workloads.proton_em_run()
+        case("etd2_proton_em_z0.05.npz", run, workloads.proton_source(run), 0.05)
+
+
+if __name__ == "__main__":
+    main(tuple(sys.argv[1:]))

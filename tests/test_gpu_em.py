@@ -87,3 +87,25 @@ def test_em_cascade_graph_equals_direct():
         states.append(g.state.copy())
     run.backend.use_hip_graphs = True
     assert np.array_equal(states[0], states[1])
+
+
+def test_em_execute_restores_initial_photons():
+    """prepare() + execute() twice: each run starts from the plan's initial
+    photon state (gam0), so both give the same photon and nuclear states."""
+    import torch
+
+    run = workloads.proton_em_run()
+    g = UHECRPropagationSolverETD2(0.07, 0.0, run)
+    g.add_source_class(workloads.proton_source(run))
+    plan = g.prepare(1e-3)
+    eb = plan["em_bufs"]
+    eb["gam0"].copy_(torch.linspace(1.0, 2.0, eb["gam0"].numel(), dtype=torch.float64)
+                     .reshape(eb["gam0"].shape))
+    outs = []
+    for _ in range(2):
+        x = g.execute(plan)
+        torch.cuda.synchronize()
+        outs.append((x.cpu().numpy().copy(), g.em_host_state(plan).copy()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.any(outs[0][1] != 0.0)

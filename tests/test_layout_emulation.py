@@ -15,6 +15,7 @@ from prince_amd.model.partial_diff import single_block
 from prince_amd.model.units import PRINCE_UNITS
 from prince_amd.solvers.layout import build_block_layout, build_tasks, fd_tables
 from prince_amd.solvers.propagation import refresh_schedule, z_grid_for
+from prince_amd.solvers.run_inputs import pair_rate_per_energy, pair_species_scale
 
 from .parity import rel_dev
 
@@ -27,8 +28,8 @@ def emulate(run, src, zi, zf, dz=1e-3, part_cost=24):
     z_grid = z_grid_for(zi, zf, dz)
     win, zs = refresh_schedule(z_grid, run.backend.update_rates_z_threshold)
     phot = np.stack([ir.photon_vector(z) for z in zs])
-    rpair = np.stack([run.pair_loss_rates_grid.rate_single(z) for z in zs])
-    B = ir.batch_matrix
+    rpair = np.stack([pair_rate_per_energy(run.pair_loss_rates_grid, z) for z in zs])
+    B = ir._batch_matrix
     m0 = B @ phot[0]
     lay = build_block_layout(ir.coupling_mat.indptr, ir.coupling_mat.indices, m0, nspec, d)
     tasks = build_tasks(lay, part_cost)
@@ -52,7 +53,7 @@ def emulate(run, src, zi, zf, dz=1e-3, part_cost=24):
     mvals = phot @ Bsel.T  # [nwin, zslots + n]
     dcoef, ddiag = fd_tables(single_block(run.cr_grid))
     E = run.cr_grid.grid
-    pscale = run.pair_loss_rates_grid.species_scale
+    pscale = pair_species_scale(run.pair_loss_rates_grid, run.spec_man)
     g = src.injection_grid.reshape(nspec, d)
     x = np.zeros((nspec, d))
     lanes = np.arange(d)

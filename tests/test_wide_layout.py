@@ -23,7 +23,7 @@ from prince_amd.solvers.layout import (
 
 def _layout(run, z=1.0):
     ir = run.int_rates
-    m0 = ir.batch_matrix @ ir.photon_vector(z)
+    m0 = ir._batch_matrix @ ir.photon_vector(z)
     d, nspec = run.cr_grid.d, run.spec_man.nspec
     cm = ir.coupling_mat
     lay = build_block_layout(cm.indptr, cm.indices, m0, nspec, d)
