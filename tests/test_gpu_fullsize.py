@@ -128,12 +128,17 @@ def test_off_diagonal_pattern_equals_oracle_split(photohad):
     # every kept entry has its own slot of the block-band layout
     assert np.unique(lay.off_slot).size == lay.off_slot.size
     assert lay.off_slot.size == 0 or lay.off_slot.max() < lay.zslots
-    # the device's window-0 values equal the host fold on the kept entries
-    m0 = plan["bufs"]["mvals"]
+    # the device's window-0 values equal the host fold: off-diagonal entries at
+    # their block-band slots, the diagonal in the window's first n values
+    wl = plan["layout"]
     s.fold_windows(plan)
-    vals = m0[0].cpu().numpy()
+    vals = plan["bufs"]["mvals"][0].cpu().numpy()
     if M_off.nnz:
-        np.testing.assert_allclose(vals[lay.off_slot], M.data[lay.off_to_M], rtol=1e-12, atol=0)
+        np.testing.assert_allclose(vals[wl.fwd_off + lay.off_slot], M.data[lay.off_to_M],
+                                   rtol=1e-12, atol=0)
+    diag = vals[wl.diag_off:wl.diag_off + n]
+    ref_diag = M.diagonal()
+    np.testing.assert_allclose(diag, ref_diag, rtol=1e-12, atol=1e-12 * np.abs(ref_diag).max())
     assert isinstance(M_off, sp.csr_matrix)
     print(f"photohad={photohad}: {M_off.nnz} off-diagonal entries equal")
     s.close()
