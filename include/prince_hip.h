@@ -255,6 +255,24 @@ int32_t prh_gather_rows(const double* src, const int64_t* sel, int64_t nrows, in
 int32_t prh_etd2_steps(const prh_operator* op, const prh_schedule* sch, const prh_state* st,
                        int32_t k_begin, int32_t k_end, void* stream);
 
+/* Allocation-free variant of prh_etd2_steps.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * hipGraph stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The caller passes the operator, the schedule and the state x
+ * ([nspec][K][d], updated in place); the stage vectors a and F(x) and the
+ * dense-row chunk partials come from one static device array per GPU of
+ * PRH_SCRATCH_BYTES (held until the process ends), whose address is looked up
+ * once per device on the first call (the device current at the call is the
+ * stream's device).  Needs 256 + 16 * nspec * K * d bytes plus
+ * 8 * ndchunk * ndense * K * d with dense-row chunk tasks, else the call
+ * fails with an error naming both sizes.  Narrow layout, no EM sector, no
+ * step recording.  It allocates nothing and keeps nothing between calls;
+ * calls that overlap on one GPU (other streams, other threads) share the
+ * array and race. */
+#define PRH_SCRATCH_BYTES (64LL << 20)
+int32_t prh_etd2_steps_noalloc(const prh_operator* op, const prh_schedule* sch, double* x,
+                               int32_t k_begin, int32_t k_end, void* stream);
+
 /* One stage (1 or 2) of ETD2 step `step`: the same kernels as
  * prh_etd2_steps, launched individually (per-kernel timing, debugging). */
 int32_t prh_etd2_stage(const prh_operator* op, const prh_schedule* sch, const prh_state* st,

@@ -18,6 +18,7 @@ EXPORTED = (
     "prh_fold_gemm",
     "prh_gather_rows",
     "prh_etd2_steps",
+    "prh_etd2_steps_noalloc",
     "prh_etd2_stage",
     "prh_graph_create",
     "prh_graph_run",
@@ -33,6 +34,8 @@ EXPORTED = (
 )
 
 WG_WAVES = 16
+#: size of the static device scratch of prh_etd2_steps_noalloc (PRH_SCRATCH_BYTES)
+SCRATCH_BYTES = 64 << 20
 
 PRH_F_PARTIAL_DIFF = 1
 PRH_F_INJECTION = 2
@@ -220,6 +223,15 @@ def load():
         ctypes.POINTER(PrhOperator),
         ctypes.POINTER(PrhSchedule),
         ctypes.POINTER(PrhState),
+        _i32,
+        _i32,
+        _p,
+    ]
+    lib.prh_etd2_steps_noalloc.restype = _i32
+    lib.prh_etd2_steps_noalloc.argtypes = [
+        ctypes.POINTER(PrhOperator),
+        ctypes.POINTER(PrhSchedule),
+        _p,
         _i32,
         _i32,
         _p,

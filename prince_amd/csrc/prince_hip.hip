@@ -22,6 +22,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/prince_hip.h"
@@ -2126,6 +2127,38 @@ void launch_adjoint_two(const prh_operator& op, const prh_schedule& sch, const p
 
 }  // namespace
 
+// Static device scratch of prh_etd2_steps_noalloc: [256 B step counter]
+// [a: nspec K d][F(x): nspec K d][dense-row chunk partials].  Every GPU holds
+// its own copy at its own address; host code uses only the looked-up address.
+static __device__ __align__(256) double scratch[PRH_SCRATCH_BYTES / 8];
+
+namespace {
+
+constexpr int PRH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[PRH_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+int32_t scratch_on_current_device(double** out) {
+  int dev = -1;
+  if (int32_t e = check_hip(hipGetDevice(&dev), "prh_etd2_steps_noalloc: hipGetDevice")) return e;
+  if (dev < 0 || dev >= PRH_MAX_DEVICES)
+    return set_error("prh_etd2_steps_noalloc: device index outside [0, 64)");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    if (int32_t e = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                              "prh_etd2_steps_noalloc: hipGetSymbolAddress(scratch)"))
+      return e;
+    if (!p) return set_error("prh_etd2_steps_noalloc: null scratch address");
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+}  // namespace
+
 struct prh_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -2180,6 +2213,34 @@ int32_t prh_etd2_steps(const prh_operator* op, const prh_schedule* sch, const pr
     if (st->em) launch_em(*op, *sch, *st, k, 0, s);
   }
   return check_hip(hipGetLastError(), "prh_etd2_steps launch");
+}
+
+int32_t prh_etd2_steps_noalloc(const prh_operator* op, const prh_schedule* sch, double* x,
+                               int32_t k_begin, int32_t k_end, void* stream) {
+  if (!op || !sch || !x) return set_error("prh_etd2_steps_noalloc: null argument");
+  if (op->wide) return set_error("prh_etd2_steps_noalloc: narrow layout only (wide = 0)");
+  if (op->nspec <= 0 || op->K <= 0 || op->d <= 0)
+    return set_error("prh_etd2_steps_noalloc: empty operator");
+  const int64_t panel = (int64_t)op->nspec * op->K * op->d;
+  const int64_t dpart = op->ndense > 0 ? (int64_t)op->ndchunk * op->ndense * op->K * op->d : 0;
+  if (dpart < 0) return set_error("prh_etd2_steps_noalloc: negative dense-row sizes");
+  const int64_t need = 256 + 8 * (2 * panel + dpart);
+  if (need > PRH_SCRATCH_BYTES)
+    return set_error("prh_etd2_steps_noalloc: needs " + std::to_string(need) +
+                     " bytes of scratch, the static array holds " +
+                     std::to_string((long long)PRH_SCRATCH_BYTES));
+  double* base = nullptr;
+  if (int32_t e = scratch_on_current_device(&base)) return e;
+  prh_state st{};
+  st.x = x;
+  st.step_ctl = reinterpret_cast<int32_t*>(base);
+  st.a = base + 32;
+  st.fphi = st.a + panel;
+  st.dpart = dpart > 0 ? st.fphi + panel : nullptr;
+  // every stage-1 task writes a and F(x) of its row before stage 2 reads
+  // them, chunk tasks write their partials before the finishing launch, and
+  // direct launches do not read the step counter: nothing needs clearing
+  return prh_etd2_steps(op, sch, &st, k_begin, k_end, stream);
 }
 
 int32_t prh_etd2_stage(const prh_operator* op, const prh_schedule* sch, const prh_state* st,

@@ -88,3 +88,20 @@ def test_abi_rejects_bad_wide_operators_without_touching_the_gpu():
     adj = _lib.PrhAdjoint(lam=ptr, lam_a=ptr, u=ptr, v=ptr)
     assert lib.prh_adjoint_step(ctypes.byref(op), ctypes.byref(sch), ctypes.byref(adj), 0, None) != 0
     assert b"narrow layout" in lib.prh_last_error()
+
+
+def test_noalloc_entry_point_rejects_before_touching_the_gpu():
+    """Null arguments, the wide layout and a scratch need beyond the static
+    array are refused on the host, before any device call."""
+    import ctypes
+
+    lib = _lib.load()
+    op = _lib.PrhOperator(nspec=305, d=64, K=256, wide=0)
+    sch = _lib.PrhSchedule(nsteps=1)
+    assert lib.prh_etd2_steps_noalloc(None, ctypes.byref(sch), None, 0, 1, None) != 0
+    assert b"null" in lib.prh_last_error()
+    rc = lib.prh_etd2_steps_noalloc(ctypes.byref(op), ctypes.byref(sch), 8, 0, 1, None)
+    assert rc != 0 and b"bytes of scratch" in lib.prh_last_error()
+    op.wide = 1
+    assert lib.prh_etd2_steps_noalloc(ctypes.byref(op), ctypes.byref(sch), 8, 0, 1, None) != 0
+    assert b"narrow" in lib.prh_last_error()
