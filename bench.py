@@ -55,6 +55,8 @@ def parse():
     p.add_argument("--adjoint-batch", type=int, default=64)
     p.add_argument("--no-uneven", action="store_true")
     p.add_argument("--no-em", action="store_true")
+    p.add_argument("--no-uneven-em", action="store_true")
+    p.add_argument("--uneven-em-batch", type=int, default=1024)
     p.add_argument("--em-batch", type=int, default=512)
     p.add_argument("--no-autotune", action="store_true")
     p.add_argument("--uneven-batch", type=int, default=1024)
@@ -311,6 +313,69 @@ def uneven_leg(run, K, torch, log, seed=0):
     }
 
 
+def uneven_em_leg(K, torch, log, seed=0):
+    """The uneven batch with the co-evolved EM cascade (K columns, z_max ~
+    U[0.1, 5], EBL scale ~ U[0.6, 1.4]): the cascade transfer and Bethe-Heitler
+    shapes of every column window are built on the device before the march.
+    Reports the host setup, the device build of the window operators and one
+    timed device march; two columns are checked against the same columns run
+    as one-column batches."""
+    from prince_amd import workloads
+    from prince_amd.solvers.uneven import UnevenBatchSolverETD2
+
+    run = workloads.nuclear_em_run()
+    rng = np.random.default_rng(seed + 1)
+    z_max = rng.uniform(0.1, 5.0, size=K)
+    scale = rng.uniform(0.6, 1.4, size=K)
+    srcs = workloads.batch_sources(run, K, seed=seed)
+    g = UnevenBatchSolverETD2(z_max, run, ebl_scale=scale)
+    for k in range(K):
+        g[k].add_source_class(srcs[k])
+    t0 = time.perf_counter()
+    plan = g.prepare(1e-3)
+    t_setup = time.perf_counter() - t0
+    t_build = g.timings.get("em_build_s", 0.0)
+    g.execute(plan, 0, min(128, plan["nsteps"]))  # warm-up; captures the chunk graphs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.execute(plan)
+    torch.cuda.synchronize()
+    t_run = time.perf_counter() - t0
+    iters = plan["em_bufs"]["iters"].cpu().numpy()
+    nspec, d, de = run.spec_man.nspec, run.cr_grid.d, run.em_grid.d
+    host = plan["x"].cpu().numpy().transpose(0, 2, 1).reshape(nspec * d, K)
+    gam = plan["em_bufs"]["gam"].cpu().numpy()
+    state = np.empty((nspec * d + de, K))
+    state[:, plan["order"]] = np.concatenate([host, gam], axis=0)
+    devs = []
+    for k in (int(np.argmin(z_max)), int(np.argsort(z_max)[K // 2])):
+        one = UnevenBatchSolverETD2(z_max[k:k + 1], run, ebl_scale=scale[k:k + 1])
+        one[0].add_source_class(srcs[k])
+        one.solve(1e-3)
+        ref = np.concatenate([one.state[:nspec * d, 0], one.state[-de:, 0]])
+        for part in (slice(0, nspec * d), slice(nspec * d, None)):
+            r, t = ref[part], state[part, k]
+            m = np.abs(r) > 1e-30 * np.abs(r).max()
+            devs.append(float(np.max(np.abs(t[m] - r[m]) / np.abs(r[m]))))
+    steps = int(plan["col_steps"].sum())
+    del g, plan
+    torch.cuda.empty_cache()
+    log(f"uneven+EM K={K}: {steps} column steps, {len(iters)} windows; setup {t_setup:.1f}s of which "
+        f"device window operators {t_build:.1f}s; device march {t_run*1e3:.0f} ms -> "
+        f"{K/t_run:.1f} solves/s ({K/(t_run + t_build):.1f} with the operator builds); "
+        f"column check {max(devs):.2e}")
+    return {
+        "K": K, "z_max_range": [0.1, 5.0], "ebl_scale_range": [0.6, 1.4],
+        "windows": int(len(iters)), "column_steps_total": steps,
+        "setup_s": round(t_setup, 2), "em_operator_build_s": round(t_build, 2),
+        "em_operator_builds_per_s": round(len(iters) / t_build, 1) if t_build > 0 else None,
+        "cascade_iterations_mean": float(np.mean(iters)), "cascade_iterations_max": int(iters.max()),
+        "ms_per_batch": round(t_run * 1e3, 1), "solves_per_s": round(K / t_run, 2),
+        "solves_per_s_with_operator_builds": round(K / (t_run + t_build), 2),
+        "column_max_rel_dev_vs_one_column_batch": max(devs),
+    }
+
+
 def em_leg(K, torch, log, seed=0):
     """Nuclear run with the co-evolved EM cascade (photons, e+-), K draws, z 3 -> 0:
     host setup (cascade transfers of the 288 windows, Bethe-Heitler shapes) and one
@@ -335,11 +400,100 @@ def em_leg(K, torch, log, seed=0):
     gam = g.em_host_state(plan)[2 * run.em_grid.d:]
     E = run.em_grid.grid
     e_ph = float(np.mean((E * np.gradient(E))[:, None] * gam, axis=0).sum())
-    log(f"EM cascade K={K}: setup {t_setup:.1f}s (cascade transfers for {plan['nwin']} windows), "
-        f"device {t_run*1e3:.0f} ms -> {K/t_run:.1f} solves/s")
-    return {"K": K, "windows": plan["nwin"], "setup_host_s": round(t_setup, 1),
+    # two columns against one-column solves of the plain path (nuclear and photons)
+    from prince_amd.solvers.propagation import UHECRPropagationSolverETD2
+
+    n, de = run.dim_states, run.em_grid.d
+    full = g._finalize_state(plan["x"])
+    devs = []
+    for k in (0, K - 1):
+        one = UHECRPropagationSolverETD2(3.0, 0.0, run)
+        one.add_source_class(srcs[k])
+        one.solve(1e-3)
+        for part in (slice(0, n), slice(n + 2 * de, n + 3 * de)):
+            r, t = one.state[part], full[part, k]
+            m = np.abs(r) > 1e-30 * np.abs(r).max()
+            devs.append(float(np.max(np.abs(t[m] - r[m]) / np.abs(r[m]))))
+    log(f"EM cascade K={K}: setup {t_setup:.1f}s (cascade transfers for {plan['nwin']} windows "
+        f"built on the device), device {t_run*1e3:.0f} ms -> {K/t_run:.1f} solves/s; "
+        f"column check vs plain path {max(devs):.2e}")
+    return {"K": K, "windows": plan["nwin"], "setup_s": round(t_setup, 2),
             "ms_per_batch": round(t_run * 1e3, 1), "solves_per_s": round(K / t_run, 2),
-            "mean_photon_energy_density": e_ph, "cascade_iterations_max": int(max(g.em_iterations))}
+            "mean_photon_energy_density": e_ph, "cascade_iterations_max": int(max(g.em_iterations)),
+            "column_max_rel_dev_vs_plain": max(devs)}
+
+
+def host_cpu():
+    """(threads available to this process, CPU model name)."""
+    try:
+        nthreads = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        nthreads = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return nthreads, model
+
+
+def cpu_baseline(run, srcs, zmax, nsteps, K, args, log):
+    """The oracle (scipy CSR SpMV single-threaded as the reference's scipy
+    backend, the dense fold through numpy/BLAS on every host thread): full
+    plain A <= 56 solves (median of up to 3 after a warm-up) -- the batch of K
+    is K such sequential solves -- and a K-column multi-RHS sample whose steps
+    after the first window are extrapolated to the full schedule."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle.etd2_cpu import OracleETD2
+    from prince_amd import workloads
+
+    nthreads, model = host_cpu()
+    with threadpool_limits(limits=nthreads):
+        times = []
+        for rep in range(4):
+            o = OracleETD2(zmax, 0.0, run)
+            o.add_source_class(workloads.silicon_source(run))
+            t0 = time.perf_counter()
+            o.solve(1e-3)
+            times.append(time.perf_counter() - t0)
+            if rep == 0 and times[0] > 40.0:  # slow host: one timed solve after the warm-up
+                o.solve(1e-3)
+                times.append(time.perf_counter() - t0 - times[0])
+                break
+        t_plain = float(np.median(times[1:]))
+        # multi-RHS sample: first window (split and index maps) timed apart
+        ns = args.cpu_sample_steps
+        om = OracleETD2(zmax, zmax - ns * 1e-3, run, K=K)
+        for k in range(K):
+            om.add_source_class(srcs[k], k)
+        marks = []
+        t0 = time.perf_counter()
+        om.solve(1e-3, step_callback=lambda k_, s_: marks.append(time.perf_counter()))
+        first = marks[0] - t0
+        per_step = (marks[-1] - marks[0]) / (len(marks) - 1)
+    t_multi = first + per_step * (nsteps - 1)
+    log(f"cpu oracle ({nthreads} threads, {model}): plain solve {t_plain:.2f} s (median of "
+        f"{len(times) - 1}); K={K} multi-RHS sample {len(marks)} steps -> {t_multi:.1f} s per batch")
+    return {
+        "value": round(1.0 / t_plain, 4),
+        "unit": "solves/s",
+        "cores": nthreads,
+        "kind": "port",
+        "cpu_model": model,
+        "sample": f"FP64 oracle, full plain A<=56 solves z {zmax:g}->0 ({nsteps} steps), median of "
+        f"{len(times) - 1} after a warm-up, dense fold on {nthreads} BLAS threads, CSR SpMV "
+        f"single-threaded; the K={K} batch counts as {K} sequential solves",
+        "plain_solve_s": round(t_plain, 3),
+        "multi_rhs_solves_per_s": round(K / t_multi, 4),
+        "multi_rhs_sample": f"K={K} columns, {len(marks)} of {nsteps} steps: first step "
+        f"(first-window fold, split and index maps) {first:.2f} s, then {per_step*1e3:.0f} ms per "
+        "step, extrapolated by step count",
+    }
 
 
 def main():
@@ -415,6 +569,17 @@ def main():
     plain_sps = 1.0 / t_plain
     log(f"batched: {ms_per:.1f} ms/solve-batch -> {batched_sps:.1f} solves/s")
 
+    # ---------------- end to end: solve() including the host setup (prepare)
+    t0 = time.perf_counter()
+    multi.solve(1e-3)
+    t_e2e = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plain.solve(1e-3)
+    t_e2e_plain = time.perf_counter() - t0
+    log(f"end to end solve(): batched {t_e2e:.2f} s -> {K / t_e2e:.1f} solves/s; "
+        f"plain {t_e2e_plain:.3f} s")
+    bplan = multi._plan
+
     # ---------------- parity of the batched columns vs the plain path
     bstate = multi._finalize_state(bplan["x"])
     check_cols = [0, K // 3, K - 1]
@@ -482,35 +647,15 @@ def main():
     if not args.no_em:
         em = em_leg(args.em_batch, torch, log)
 
-    # ---------------- CPU baseline: oracle on a bounded sample of the batched workload
+    # ---------------- uneven batch with the EM cascade (C3 as specified)
+    uneven_em = None
+    if not args.no_uneven_em:
+        uneven_em = uneven_em_leg(args.uneven_em_batch, torch, log)
+
+    # ---------------- CPU baseline: the FP64 oracle on the host cores
     cpu = None
     if not args.no_cpu:
-        from threadpoolctl import threadpool_limits
-
-        from oracle.etd2_cpu import OracleETD2
-
-        ns = args.cpu_sample_steps
-        zend = zmax - ns * 1e-3
-        o = OracleETD2(zmax, zend, run, K=K)
-        for k in range(K):
-            o.add_source_class(srcs[k], k)
-        with threadpool_limits(limits=1):
-            t0 = time.perf_counter()
-            o.solve(1e-3)
-            tc = time.perf_counter() - t0
-        nsam = len(o.z_grid) - 1
-        per_step = tc / nsam
-        cpu_sps = K / (per_step * nsteps)
-        cpu = {
-            "value": round(cpu_sps, 4),
-            "unit": "solves/s",
-            "cores": 1,
-            "kind": "port",
-            "sample": f"CPU FP64 oracle, K={K} batched columns, first {nsam} of {nsteps} steps "
-            f"(z {zmax:g} -> {zend:.3f}, {len(o.refresh_zs)} rate refreshes) in {tc:.1f}s, "
-            "extrapolated by step count",
-        }
-        log(f"cpu oracle sample: {tc:.1f}s for {nsam} steps -> {cpu_sps:.3f} solves/s")
+        cpu = cpu_baseline(run, srcs, zmax, nsteps, K, args, log)
 
     out = {
         "metric": METRIC,
@@ -539,10 +684,14 @@ def main():
         "steps_per_s_batched": round(nsteps * args.steps / t_batch, 1),
         "steps_per_s_plain": round(nsteps / t_plain, 1),
         "batched_column_max_rel_dev_vs_plain": max(col_dev),
+        "end_to_end_solve_s_batched": round(t_e2e, 3),
+        "end_to_end_solves_per_s_batched": round(K / t_e2e, 2),
+        "end_to_end_solve_s_plain": round(t_e2e_plain, 4),
         "adjoint_forward_ratio": adjoint["ratio"] if adjoint else None,
         "adjoint": adjoint,
         "uneven": uneven,
         "em_cascade": em,
+        "uneven_em": uneven_em,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,

@@ -235,7 +235,9 @@ const char* prh_last_error(void);
  * for w < nwin, r < nrows.  Replaces the per-window dense fold
  * M.data = B @ n_gamma of src/prince_cr/interaction_rates.py:1051-1061
  * (host) and :930-964 (GPU), and the split fold of :824-847, evaluated for
- * every refresh window of a solve in one launch. */
+ * every refresh window of a solve in one launch.  dph <= 4096 (above 128 the
+ * kernel rows are re-read per window tile instead of held in registers: the
+ * cascade's tensor contractions, cascade/cascade.py:78-188). */
 int32_t prh_fold_gemm(const double* kern, int64_t nrows, int32_t dph, const double* phot,
                       int32_t nwin, double* out, int64_t out_stride, void* stream);
 
@@ -318,7 +320,12 @@ int32_t prh_fold_rows(const double* kern, int64_t nrows, int32_t dph, const doub
 
 /* Both ETD2 stages of step `step` of the uneven batch (narrow layout, kt 1/2/4).
  * Each column follows its own reference solve (propagation.py:813-890 with
- * etd2.py:412-439); finished columns are skipped. */
+ * etd2.py:412-439); finished columns are skipped.  With st->em set, the EM
+ * step of every live column follows (its own window's T_gamma, T_electron,
+ * R_BH and Bethe-Heitler weights: tables indexed by the column windows of
+ * un->win, em->nwin == un->nwin; the photon state em->gam [d_em][K] is
+ * updated in place, em->gam2 and em->wbh are not used): the reference's step
+ * hook (propagation.py:867-878) per column. */
 int32_t prh_uneven_step(const prh_operator* op, const prh_uneven* un, const prh_state* st,
                         int32_t step, void* stream);
 
@@ -343,6 +350,51 @@ int32_t prh_graph_create(const prh_operator* op, const prh_schedule* sch, const 
                          int32_t chunk, prh_graph** out);
 int32_t prh_graph_run(prh_graph* g, int32_t k_begin, int32_t k_end, void* stream);
 int32_t prh_graph_destroy(prh_graph* g);
+
+/* ---------------------------------------------------------------------------
+ * EM-cascade window operators built on the device.
+ * ------------------------------------------------------------------------- */
+#define PRH_BH_MAX 64 /* target-photon points of the Bethe-Heitler kernel */
+
+/* Field-free cascade data of one EM grid (device pointers). */
+typedef struct prh_cascade {
+  int32_t n;           /* EM bins, <= 144 */
+  int32_t d;           /* nucleon bins of the Bethe-Heitler source, <= 64 */
+  int32_t nb;          /* target-photon points of the Bethe-Heitler kernel, <= PRH_BH_MAX */
+  int32_t max_scatter; /* iteration cap of the kinetic cascade */
+  double tol;          /* energy left in G and e, relative to each column's input, at convergence */
+  double cool_floor;   /* a column whose IC emission carries less than this fraction of E cools */
+  double bh_emin;      /* lowest nucleon energy with a Bethe-Heitler source */
+  const double* E;         /* [n] EM grid centres */
+  const double* dE;        /* [n] bin widths (gradient of E) */
+  const double* pref_emis; /* [n] IC emission prefactor (3/4) sigma_T c / gamma^2 */
+  const double* pref_pair; /* [n] pair-injection prefactor */
+  const int32_t* cic_b;    /* [n][n] (k, j): lower bin of E_j - E_k, -1 off the grid */
+  const double* cic_f;     /* [n][n] weight of that lower bin */
+  const double* Ecr;       /* [d] nucleon grid */
+  const double* eps_b;     /* [nb] Bethe-Heitler target-photon grid */
+  const double* kern_b;    /* [n][d][nb] Bethe-Heitler kernel */
+} prh_cascade;
+
+/* Kinetic single-scatter cascade transfer of nwin windows (one 16-wave
+ * workgroup each).  contr [nwin][cstride]: the window's target densities
+ * contracted with the field-free tensors (prh_fold_gemm): IC emission
+ * [n][n], pair injection [n][n], IC power [n], IC rate [n], sum w n,
+ * sum w eps n; dtau [nwin][n] the opacity per unit length, dl [nwin] the
+ * path length of one step.  work: 8 * 144 * 144 doubles per window.
+ * Outputs T_gamma, T_electron [nwin][n][n] and the iteration counts.
+ * Replaces the per-window host builds of the reference's cascade
+ * (cascade/cascade.py:331-407, batched form 444-589). */
+int32_t prh_cascade_transfer(const prh_cascade* cs, const double* contr, int64_t cstride,
+                             const double* dtau, const double* dl, int32_t nwin, double* work,
+                             int64_t work_bytes, double* tg, double* te, int32_t* iters,
+                             void* stream);
+
+/* Bethe-Heitler e+- shapes R [nwin][n][d] from the target densities at the
+ * kernel's photon grid [nwin][nb] (cascade/bethe_heitler.py:219-265,
+ * batched 319-351). */
+int32_t prh_cascade_bh(const prh_cascade* cs, const double* nb_dens, int32_t nwin, double* rbh,
+                       void* stream);
 
 #ifdef __cplusplus
 }

@@ -31,6 +31,8 @@ EXPORTED = (
     "prh_fold_rows",
     "prh_uneven_step",
     "prh_uneven_graph_create",
+    "prh_cascade_transfer",
+    "prh_cascade_bh",
 )
 
 WG_WAVES = 16
@@ -196,6 +198,27 @@ class HipLibraryError(RuntimeError):
     pass
 
 
+class PrhCascade(ctypes.Structure):
+    _fields_ = [
+        ("n", _i32),
+        ("d", _i32),
+        ("nb", _i32),
+        ("max_scatter", _i32),
+        ("tol", ctypes.c_double),
+        ("cool_floor", ctypes.c_double),
+        ("bh_emin", ctypes.c_double),
+        ("E", _p),
+        ("dE", _p),
+        ("pref_emis", _p),
+        ("pref_pair", _p),
+        ("cic_b", _p),
+        ("cic_f", _p),
+        ("Ecr", _p),
+        ("eps_b", _p),
+        ("kern_b", _p),
+    ]
+
+
 def load():
     """Load (once) and return the ctypes library handle."""
     global _LIB
@@ -236,6 +259,11 @@ def load():
         _i32,
         _p,
     ]
+    lib.prh_cascade_transfer.restype = _i32
+    lib.prh_cascade_transfer.argtypes = [ctypes.POINTER(PrhCascade), _p, _i64, _p, _p, _i32, _p,
+                                         _i64, _p, _p, _p, _p]
+    lib.prh_cascade_bh.restype = _i32
+    lib.prh_cascade_bh.argtypes = [ctypes.POINTER(PrhCascade), _p, _i32, _p, _p]
     lib.prh_etd2_stage.restype = _i32
     lib.prh_etd2_stage.argtypes = [
         ctypes.POINTER(PrhOperator),

@@ -112,6 +112,45 @@ __global__ void __launch_bounds__(THREADS) fold_gemm_mfma(const double* __restri
   }
 }
 
+// Same product for a contraction length above 128 (the cascade tensors: 400
+// and 256 target-photon points): the kernel rows are re-read from cache per
+// 16-window tile instead of held in registers.
+__global__ void __launch_bounds__(THREADS) contract_rows_mfma(const double* __restrict__ kern,
+                                                              int64_t nrows, int kdim,
+                                                              const double* __restrict__ vec,
+                                                              int nvec, double* __restrict__ out,
+                                                              int64_t out_stride) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t wave = (int64_t)blockIdx.x * WAVES_PER_BLOCK + wave_id();
+  const int64_t r0 = wave * 16;
+  if (r0 >= nrows) return;
+  const int col = lane & 15;
+  const int kq = lane >> 4;
+  const int64_t r = r0 + col;
+  const double* kr = kern + (r < nrows ? r : nrows - 1) * (int64_t)kdim;
+  for (int w0 = 0; w0 < nvec; w0 += 16) {
+    double4_t acc = {0.0, 0.0, 0.0, 0.0};
+    const int wa = w0 + col;
+    const double* va = vec + (int64_t)(wa < nvec ? wa : nvec - 1) * kdim;
+    for (int k0 = 0; k0 < kdim; k0 += 16) {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + 4 * u + kq;
+        a[u] = (wa < nvec && k < kdim) ? va[k] : 0.0;
+        b[u] = (r < nrows && k < kdim) ? kr[k] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int w = w0 + kq + 4 * v;
+      if (w < nvec && r < nrows) out[(int64_t)w * out_stride + r] = acc[v];
+    }
+  }
+}
+
 __global__ void gather_rows_kernel(const double* __restrict__ src, const int64_t* __restrict__ sel,
                                    int64_t nrows, int ncols, double* __restrict__ dst) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1845,6 +1884,116 @@ void launch_uneven(const prh_operator& op, const prh_uneven& un, const prh_state
   }
 }
 
+// EM sector of the uneven batch: one workgroup per column k that has step
+// `step` (win >= 0), after both nuclear stages.  The column's own window w
+// selects T_gamma, T_electron, R_BH and the Bethe-Heitler weights; its own
+// h, dl/dz and H cm2sec drive the photon ETD2 step.  The photon state
+// [d_em][K] is updated in place (the whole column lives in one workgroup).
+// Same sums in the same order as em_bh_weights_bins + em_step_kernel.
+__global__ void __launch_bounds__(256) em_uneven_kernel(prh_operator op, prh_uneven un,
+                                                       prh_state st, prh_em em, int step) {
+  __shared__ double part[4][WAVE];
+  __shared__ double wb[WAVE];
+  __shared__ double g0[PRH_EM_MAX], ga[PRH_EM_MAX], lep[PRH_EM_MAX];
+  const int K = op.K;
+  const int d = op.d;
+  const int de = em.d_em;
+  const int k = blockIdx.x;
+  const int w = un.win[(int64_t)step * K + k];
+  if (w < 0) return;  // uniform per workgroup: the column has finished
+  const int tid = threadIdx.x;
+  const int lane = tid & (WAVE - 1);
+  const int wv = wave_id();
+  // Bethe-Heitler weights of the column: 4 waves over interleaved species
+  {
+    const bool act = lane < d;
+    const int ns = op.nspec;
+    const int64_t sS = (int64_t)K * d;
+    const double* xp = st.x + (int64_t)k * d + (act ? lane : 0);
+    double acc = 0.0;
+    int s = wv;
+    for (; s + 28 < ns; s += 32) {
+      double xv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) xv[u] = act ? xp[(int64_t)(s + 4 * u) * sS] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += em.cbh[s + 4 * u] * xv[u];
+    }
+    for (; s < ns; s += 4) acc += em.cbh[s] * (act ? xp[(int64_t)s * sS] : 0.0);
+    part[wv][lane] = acc;
+  }
+  const double hh = un.h[(int64_t)step * K + k];
+  const double dl = un.dldz[(int64_t)step * K + k];
+  const double had = un.hadia[(int64_t)step * K + k];
+  const bool pdiff = (op.flags & PRH_F_PARTIAL_DIFF) != 0;
+  for (int i = tid; i < de; i += blockDim.x) g0[i] = em.gam[(int64_t)i * K + k];
+  __syncthreads();
+  if (wv == 0 && lane < d) {
+    const double tot = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+    wb[lane] = em.bhw[(int64_t)w * d + lane] * tot;
+  }
+  auto stencil = [&](const double* y, int i) {
+    double f = 0.0;
+    if (pdiff) {
+#pragma unroll
+      for (int o = -3; o <= 3; ++o) {
+        const int j = i + o;
+        if (o == 0 || j < 0 || j >= de) continue;
+        const double c = em.dcoef[i * 7 + o + 3];
+        if (c != 0.0) f += c * ((had * em.egrid[j]) * y[j]);
+      }
+    }
+    return f * dl;
+  };
+  // photon ETD2 step, stage 1
+  double f0 = 0.0, p2 = 0.0;
+  const int i = tid;
+  if (i < de) {
+    double dv = 0.0;
+    if (pdiff) {
+      double kx = (had * em.egrid[i]) * em.ddiag[i];
+      dv = kx * dl;
+    }
+    const double hd = dv * hh;
+    const double e = exp(hd);
+    const double p1 = phi1_of(hd, e);
+    p2 = phi2_of(hd, e);
+    f0 = stencil(g0, i);
+    double a = e * g0[i];
+    double sc = p1 * f0;
+    sc = sc * hh;
+    ga[i] = a + sc;
+  }
+  __syncthreads();
+  double xn = 0.0;
+  if (i < de) {
+    const double fa = stencil(ga, i);
+    double sc = fa - f0;
+    sc = sc * hh;
+    sc = sc * p2;
+    xn = ga[i] + sc;
+    // Bethe-Heitler pairs: e_src = R_BH w, lep = e- + e+ = 2 e_src
+    const double* R = em.rbh + ((int64_t)w * de + i) * d;
+    double es = 0.0;
+    for (int j = 0; j < d; ++j) es += R[j] * wb[j];
+    lep[i] = es + es;
+  }
+  __syncthreads();
+  if (i < de) g0[i] = xn;
+  __syncthreads();
+  // cascade transfer: gam <- T_gamma gam + T_electron lep
+  if (i < de) {
+    const double* Tg = em.tg + ((int64_t)w * de + i) * de;
+    const double* Te = em.te + ((int64_t)w * de + i) * de;
+    double ag = 0.0, ae = 0.0;
+    for (int m = 0; m < de; ++m) {
+      ag += Tg[m] * g0[m];
+      ae += Te[m] * lep[m];
+    }
+    em.gam[(int64_t)i * K + k] = ag + ae;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Dense rows (protons, He-4: a block from nearly every mother).  dense_pass
 // walks the mothers once per (mother chunk, KT right-hand sides): each mother
@@ -2125,6 +2274,325 @@ void launch_adjoint_two(const prh_operator& op, const prh_schedule& sch, const p
   }
 }
 
+// ---------------------------------------------------------------------------
+// EM cascade per-window operators built on the device (reference
+// cascade/cascade.py:284-407 and 425-589, cascade/bethe_heitler.py:219-351).
+//
+// For every window w (redshift z_w, field scale s_w) the field-free tensors
+// are contracted with the target-photon densities by prh_fold_gemm (MFMA);
+// cascade_transfer_kernel then builds, per window in one 16-wave workgroup,
+// the pair matrix P, the IC emission shape Sg, the degraded-electron map De,
+// the IC cooling map Mic and the kinetic single-scatter iteration
+//   esc += pesc G;  e += P ((G - pesc G) dE);
+//   G_c = Mic (e_cold dE);  G_h = Sg (e_hot dE);  e = De (e_hot dE);
+//   esc += pesc G_c;  G = G_h + G_c - pesc G_c
+// until the energy left in G and e is below tol of each column's input; the
+// outputs are T_gamma = esc and T_electron = esc (Mic dE).  The three
+// products of a step share their right operand and run as one pass of
+// v_mfma_f64_16x16x4 tiles over the 144 x 144 padded matrices held in a
+// per-window workspace in HBM (L2-resident while the workgroup runs).
+// ---------------------------------------------------------------------------
+constexpr int CNP = 144;              // padded EM dimension (n <= 144)
+constexpr int CTL = CNP / 16;         // 9 tiles per matrix side
+constexpr int CWAVES = 16;
+constexpr int CTHREADS = CWAVES * WAVE;
+constexpr int CTILES = CTL * CTL;     // 81 output tiles
+constexpr int CTPW = (CTILES + CWAVES - 1) / CWAVES;  // tiles per wave (6)
+constexpr int64_t CMAT = (int64_t)CNP * CNP;
+enum CascadeMat { CM_P = 0, CM_SG, CM_DE, CM_MIC, CM_G, CM_E, CM_E1, CM_ESC, CM_COUNT };
+
+// rows of the contraction output per window (prh_cascade layout)
+__device__ __forceinline__ int64_t crow_emis(int) { return 0; }
+__device__ __forceinline__ int64_t crow_pair(int n) { return (int64_t)n * n; }
+__device__ __forceinline__ int64_t crow_power(int n) { return 2 * (int64_t)n * n; }
+__device__ __forceinline__ int64_t crow_rate(int n) { return 2 * (int64_t)n * n + n; }
+__device__ __forceinline__ int64_t crow_den(int n) { return 2 * (int64_t)n * n + 2 * n; }
+__device__ __forceinline__ int64_t crow_num(int n) { return 2 * (int64_t)n * n + 2 * n + 1; }
+
+// trapezoid of E * M[:, j] over E (column j, rows 0..n-1), sequential
+__device__ __forceinline__ double column_energy(const double* M, int j, int n,
+                                                const double* E) {
+  double acc = 0.0;
+  for (int i = 0; i + 1 < n; ++i) {
+    const double dx = E[i + 1] - E[i];
+    acc += dx * (E[i + 1] * M[(int64_t)(i + 1) * CNP + j] + E[i] * M[(int64_t)i * CNP + j]) / 2.0;
+  }
+  return acc;
+}
+
+__global__ void __launch_bounds__(CTHREADS) cascade_transfer_kernel(
+    prh_cascade cs, const double* __restrict__ contr, int64_t cstride,
+    const double* __restrict__ dtau, const double* __restrict__ dl, double* __restrict__ work,
+    double* __restrict__ tg, double* __restrict__ te, int32_t* __restrict__ iters) {
+  __shared__ double sE[CNP], sdE[CNP], sR[CNP], sloss[CNP], spesc[CNP], scold[CNP], sEin[CNP];
+  __shared__ double sred[CNP];
+  __shared__ int sdone;
+  const int n = cs.n;
+  const int w = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid & (WAVE - 1);
+  const int wv = wave_id();
+  const double* cw = contr + (int64_t)w * cstride;
+  double* M = work + (int64_t)w * CM_COUNT * CMAT;
+  double* P = M + CM_P * CMAT;
+  double* Sg = M + CM_SG * CMAT;
+  double* De = M + CM_DE * CMAT;
+  double* Mic = M + CM_MIC * CMAT;
+  double* G = M + CM_G * CMAT;
+  double* Ev = M + CM_E * CMAT;
+  double* E1 = M + CM_E1 * CMAT;
+  double* Esc = M + CM_ESC * CMAT;
+
+  for (int i = tid; i < CNP; i += CTHREADS) {
+    sE[i] = i < n ? cs.E[i] : 0.0;
+    sdE[i] = i < n ? cs.dE[i] : 0.0;
+  }
+  // emission into E1, pair spectrum into P, zero the rest (padding included)
+  for (int64_t t = tid; t < CMAT; t += CTHREADS) {
+    const int i = (int)(t / CNP), j = (int)(t % CNP);
+    const bool in = i < n && j < n;
+    E1[t] = in ? cs.pref_emis[j] * cw[crow_emis(n) + (int64_t)i * n + j] : 0.0;
+    P[t] = in ? cs.pref_pair[j] * cw[crow_pair(n) + (int64_t)i * n + j] : 0.0;
+    De[t] = 0.0;
+    G[t] = (in && i == j) ? 1.0 : 0.0;
+    Ev[t] = 0.0;
+    Esc[t] = 0.0;
+  }
+  __syncthreads();
+  // per column j: pair-matrix energy normalisation, IC emission rate R_j
+  for (int j = tid; j < n; j += CTHREADS) {
+    const double oe = column_energy(P, j, n, sE);
+    if (oe > 0.0)
+      for (int i = 0; i < n; ++i) P[(int64_t)i * CNP + j] = (P[(int64_t)i * CNP + j] / oe) * sE[j];
+    double r = 0.0;
+    for (int i = 0; i < n; ++i) r += E1[(int64_t)i * CNP + j] * sdE[i];
+    sR[j] = r;
+  }
+  if (tid == 0) {
+    const double den = cw[crow_den(n)];
+    const double eps_mean = den > 0.0 ? cw[crow_num(n)] / den : 0.0;
+    sred[0] = eps_mean;
+  }
+  __syncthreads();
+  const double eps_mean = sred[0];
+  for (int j = tid; j < n; j += CTHREADS) {
+    const double pw = cs.pref_emis[j] * cw[crow_power(n) + j];
+    const double rt = cs.pref_emis[j] * cw[crow_rate(n) + j];
+    sloss[j] = pw - rt * eps_mean;
+  }
+  __syncthreads();
+  // Sg, cold flag and the degraded-electron map (column j by one thread, rows
+  // in order as the host's scatter-add)
+  for (int j = tid; j < n; j += CTHREADS) {
+    const double r = sR[j];
+    double eg = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double s = r > 0.0 ? E1[(int64_t)i * CNP + j] / r : 0.0;
+      Sg[(int64_t)i * CNP + j] = s;
+      eg += (sE[i] * s) * sdE[i];
+    }
+    scold[j] = ((eg < cs.cool_floor * sE[j]) || (r <= 0.0)) ? 1.0 : 0.0;
+    if (r > 0.0) {
+      for (int k = 0; k < n; ++k) {
+        const int b = cs.cic_b[(int64_t)k * n + j];
+        const double wk = Sg[(int64_t)k * CNP + j] * sdE[k];
+        if (b < 0 || !(wk > 0.0)) continue;
+        const double f = cs.cic_f[(int64_t)k * n + j];
+        De[(int64_t)b * CNP + j] += wk * f / sdE[b];
+        De[(int64_t)(b + 1) * CNP + j] += wk * (1.0 - f) / sdE[b + 1];
+      }
+    }
+  }
+  // IC cooling map: row i cumulates emis/loss along j, relative to column i
+  for (int i = tid; i < n; i += CTHREADS) {
+    double cum = 0.0, prev = 0.0, at_i = 0.0;
+    for (int j = 0; j < n; ++j) {
+      const double l = sloss[j];
+      const double g = l > 0.0 ? E1[(int64_t)i * CNP + j] / l : 0.0;
+      if (j > 0) cum = cum + (sE[j] - sE[j - 1]) * (g + prev) / 2.0;
+      prev = g;
+      if (j == i) at_i = cum;
+      Mic[(int64_t)i * CNP + j] = j >= i ? cum - at_i : 0.0;
+    }
+    spesc[i] = exp(-dtau[(int64_t)w * n + i] * dl[w]);
+    sEin[i] = sE[i] * sdE[i];
+  }
+  for (int i = n + tid; i < CNP; i += CTHREADS) {
+    spesc[i] = 0.0;
+    sEin[i] = 0.0;
+    scold[i] = 0.0;
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += CTHREADS) {
+    const double oe = column_energy(Mic, j, n, sE);
+    if (oe > 0.0)
+      for (int i = 0; i < n; ++i) Mic[(int64_t)i * CNP + j] = (Mic[(int64_t)i * CNP + j] / oe) * sE[j];
+  }
+  __syncthreads();
+
+  const int col = lane & 15;
+  const int kq = lane >> 4;
+  int it = 0;
+  for (it = 1; it <= cs.max_scatter; ++it) {
+    // e1 = e + P ((G - pesc G) dE);  esc += pesc G
+#pragma unroll 1
+    for (int q = 0; q < CTPW; ++q) {
+      const int t = wv + CWAVES * q;
+      if (t >= CTILES) break;
+      const int r0 = (t / CTL) * 16, c0 = (t % CTL) * 16;
+      double4_t acc = {0.0, 0.0, 0.0, 0.0};
+      for (int k0 = 0; k0 < CNP; k0 += 4) {
+        const int k = k0 + kq;
+        const double a = P[(int64_t)(r0 + col) * CNP + k];
+        const double g = G[(int64_t)k * CNP + c0 + col];
+        const double b = (g - spesc[k] * g) * sdE[k];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = r0 + kq + 4 * v, j = c0 + col;
+        const int64_t o = (int64_t)i * CNP + j;
+        E1[o] = Ev[o] + acc[v];
+        Esc[o] += spesc[i] * G[o];
+      }
+    }
+    __syncthreads();
+    // G_c = Mic (e1 cold dE), G_h = Sg (e1 hot dE), e = De (e1 hot dE)
+#pragma unroll 1
+    for (int q = 0; q < CTPW; ++q) {
+      const int t = wv + CWAVES * q;
+      if (t >= CTILES) break;
+      const int r0 = (t / CTL) * 16, c0 = (t % CTL) * 16;
+      double4_t gc = {0.0, 0.0, 0.0, 0.0}, gh = gc, en = gc;
+      for (int k0 = 0; k0 < CNP; k0 += 4) {
+        const int k = k0 + kq;
+        const int64_t ra = (int64_t)(r0 + col) * CNP + k;
+        const double ek = E1[(int64_t)k * CNP + c0 + col];
+        const double yc = scold[k] != 0.0 ? ek * sdE[k] : 0.0;
+        const double yh = scold[k] != 0.0 ? 0.0 : ek * sdE[k];
+        gc = __builtin_amdgcn_mfma_f64_16x16x4f64(Mic[ra], yc, gc, 0, 0, 0);
+        gh = __builtin_amdgcn_mfma_f64_16x16x4f64(Sg[ra], yh, gh, 0, 0, 0);
+        en = __builtin_amdgcn_mfma_f64_16x16x4f64(De[ra], yh, en, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = r0 + kq + 4 * v, j = c0 + col;
+        const int64_t o = (int64_t)i * CNP + j;
+        Esc[o] += spesc[i] * gc[v];
+        G[o] = (gh[v] + gc[v]) - spesc[i] * gc[v];
+        Ev[o] = en[v];
+      }
+    }
+    __syncthreads();
+    // energy still in G and e per input column, relative to the column's input
+    double worst = -1.0;
+    for (int j = tid; j < n; j += CTHREADS) {
+      double eg = 0.0, ee = 0.0;
+      for (int i = 0; i < n; ++i) {
+        eg += sEin[i] * G[(int64_t)i * CNP + j];
+        ee += sEin[i] * Ev[(int64_t)i * CNP + j];
+      }
+      worst = fmax(worst, (eg + ee) / sEin[j]);
+    }
+    if (tid < CNP) sred[tid] = worst;
+    __syncthreads();
+    if (tid == 0) {
+      double m = sred[0];
+      for (int j = 1; j < CNP; ++j) m = fmax(m, sred[j]);
+      sdone = m < cs.tol ? 1 : 0;
+    }
+    __syncthreads();
+    if (sdone) {
+      for (int64_t t = tid; t < CMAT; t += CTHREADS) {
+        const int i = (int)(t / CNP);
+        Esc[t] += spesc[i] * G[t];
+      }
+      __syncthreads();
+      break;
+    }
+  }
+  if (tid == 0) iters[w] = it > cs.max_scatter ? cs.max_scatter : it;
+  // outputs: T_gamma = esc, T_electron = esc (Mic dE)
+  double* tgw = tg + (int64_t)w * n * n;
+  double* tew = te + (int64_t)w * n * n;
+  for (int64_t t = tid; t < (int64_t)n * n; t += CTHREADS) {
+    const int i = (int)(t / n), j = (int)(t % n);
+    tgw[t] = Esc[(int64_t)i * CNP + j];
+  }
+#pragma unroll 1
+  for (int q = 0; q < CTPW; ++q) {
+    const int t = wv + CWAVES * q;
+    if (t >= CTILES) break;
+    const int r0 = (t / CTL) * 16, c0 = (t % CTL) * 16;
+    double4_t acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < CNP; k0 += 4) {
+      const int k = k0 + kq;
+      const int j = c0 + col;
+      const double a = Esc[(int64_t)(r0 + col) * CNP + k];
+      const double b = Mic[(int64_t)k * CNP + j] * sdE[j];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int i = r0 + kq + 4 * v, j = c0 + col;
+      if (i < n && j < n) tew[(int64_t)i * n + j] = acc[v];
+    }
+  }
+}
+
+// Bethe-Heitler pair shapes R[w][i][j] (lepton bin i of the EM grid per
+// nucleon bin j): K_bh contracted over ln eps with the window's target
+// densities by the trapezoid over the points where n > 0, restricted to the
+// band E_i <= Ep_j, E_i > 1e-6 Ep_j, Ep_j >= bh_emin, and normalised per
+// column so that 2 int E R dE = Ep_j.  One workgroup per window; the column
+// pass is one thread per nucleon bin.
+__global__ void __launch_bounds__(256) cascade_bh_kernel(prh_cascade cs,
+                                                         const double* __restrict__ nb_dens,
+                                                         double* __restrict__ rbh) {
+  __shared__ double wts[PRH_BH_MAX];
+  const int n = cs.n, d = cs.d, nb = cs.nb;
+  const int w = blockIdx.x;
+  const int tid = threadIdx.x;
+  const double* nw = nb_dens + (int64_t)w * nb;
+  if (tid == 0) {
+    // trapezoid weights on ln eps over the used points, times n eps
+    for (int u = 0; u < nb; ++u) wts[u] = 0.0;
+    int prev = -1;
+    for (int u = 0; u < nb; ++u) {
+      if (!(nw[u] > 0.0)) continue;
+      if (prev >= 0) {
+        const double h = 0.5 * (log(cs.eps_b[u]) - log(cs.eps_b[prev]));
+        wts[prev] += h;
+        wts[u] += h;
+      }
+      prev = u;
+    }
+    for (int u = 0; u < nb; ++u) wts[u] *= nw[u] > 0.0 ? nw[u] * cs.eps_b[u] : 0.0;
+  }
+  __syncthreads();
+  double* R = rbh + (int64_t)w * n * d;
+  for (int64_t t = tid; t < (int64_t)n * d; t += blockDim.x) {
+    const int i = (int)(t / d), j = (int)(t % d);
+    const double e = cs.E[i], ep = cs.Ecr[j];
+    double v = 0.0;
+    if (e <= ep && e > 1e-6 * ep && ep >= cs.bh_emin) {
+      const double* kr = cs.kern_b + t * nb;
+      for (int u = 0; u < nb; ++u) v += kr[u] * wts[u];
+    }
+    R[t] = v;
+  }
+  __syncthreads();
+  for (int j = tid; j < d; j += blockDim.x) {
+    double carried = 0.0;
+    for (int i = 0; i + 1 < n; ++i)
+      carried += (cs.E[i + 1] - cs.E[i]) *
+                 (cs.E[i + 1] * R[(int64_t)(i + 1) * d + j] + cs.E[i] * R[(int64_t)i * d + j]) / 2.0;
+    carried *= 2.0;
+    const double f = carried > 0.0 ? cs.Ecr[j] / carried : 0.0;
+    for (int i = 0; i < n; ++i) R[(int64_t)i * d + j] *= f;
+  }
+}
+
 }  // namespace
 
 // Static device scratch of prh_etd2_steps_noalloc: [256 B step counter]
@@ -2177,14 +2645,19 @@ const char* prh_last_error(void) { return g_last_error.c_str(); }
 int32_t prh_fold_gemm(const double* kern, int64_t nrows, int32_t dph, const double* phot,
                       int32_t nwin, double* out, int64_t out_stride, void* stream) {
   if (nrows < 0 || nwin < 0) return set_error("prh_fold_gemm: negative size");
-  if (dph <= 0 || dph > 4 * FOLD_MAX_KSTEPS) return set_error("prh_fold_gemm: dph must be in [1, 128]");
+  if (dph <= 0 || dph > 4096) return set_error("prh_fold_gemm: dph must be in [1, 4096]");
   if (out_stride < nrows) return set_error("prh_fold_gemm: out_stride < nrows");
   if (nrows == 0 || nwin == 0) return 0;
   if (!kern || !phot || !out) return set_error("prh_fold_gemm: null pointer");
   const int64_t waves = (nrows + 15) / 16;
   const int64_t blocks = (waves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
-  hipLaunchKernelGGL(fold_gemm_mfma, dim3((unsigned)blocks), dim3(THREADS), 0,
-                     (hipStream_t)stream, kern, nrows, (int)dph, phot, (int)nwin, out, out_stride);
+  if (blocks > 0x7fffffff) return set_error("prh_fold_gemm: too many rows");
+  if (dph <= 4 * FOLD_MAX_KSTEPS)
+    hipLaunchKernelGGL(fold_gemm_mfma, dim3((unsigned)blocks), dim3(THREADS), 0,
+                       (hipStream_t)stream, kern, nrows, (int)dph, phot, (int)nwin, out, out_stride);
+  else
+    hipLaunchKernelGGL(contract_rows_mfma, dim3((unsigned)blocks), dim3(THREADS), 0,
+                       (hipStream_t)stream, kern, nrows, (int)dph, phot, (int)nwin, out, out_stride);
   return check_hip(hipGetLastError(), "prh_fold_gemm launch");
 }
 
@@ -2391,6 +2864,42 @@ int32_t prh_graph_destroy(prh_graph* g) {
 }
 
 
+int32_t prh_cascade_transfer(const prh_cascade* cs, const double* contr, int64_t cstride,
+                             const double* dtau, const double* dl, int32_t nwin, double* work,
+                             int64_t work_bytes, double* tg, double* te, int32_t* iters,
+                             void* stream) {
+  if (!cs || !contr || !dtau || !dl || !work || !tg || !te || !iters)
+    return set_error("prh_cascade_transfer: null argument");
+  if (cs->n < 2 || cs->n > CNP) return set_error("prh_cascade_transfer: n must be in [2, 144]");
+  if (!cs->E || !cs->dE || !cs->pref_emis || !cs->pref_pair || !cs->cic_b || !cs->cic_f)
+    return set_error("prh_cascade_transfer: null field-free table");
+  if (cs->max_scatter < 1) return set_error("prh_cascade_transfer: max_scatter must be >= 1");
+  if (nwin < 0) return set_error("prh_cascade_transfer: negative window count");
+  if (nwin == 0) return 0;
+  const int64_t rows = 2 * (int64_t)cs->n * cs->n + 2 * cs->n + 2;
+  if (cstride < rows) return set_error("prh_cascade_transfer: cstride below the contraction rows");
+  if (work_bytes < (int64_t)nwin * CM_COUNT * CMAT * 8)
+    return set_error("prh_cascade_transfer: workspace below 8 * 144 * 144 doubles per window");
+  hipLaunchKernelGGL(cascade_transfer_kernel, dim3((unsigned)nwin), dim3(CTHREADS), 0,
+                     (hipStream_t)stream, *cs, contr, cstride, dtau, dl, work, tg, te, iters);
+  return check_hip(hipGetLastError(), "prh_cascade_transfer launch");
+}
+
+int32_t prh_cascade_bh(const prh_cascade* cs, const double* nb_dens, int32_t nwin, double* rbh,
+                       void* stream) {
+  if (!cs || !nb_dens || !rbh) return set_error("prh_cascade_bh: null argument");
+  if (cs->n < 2 || cs->n > CNP || cs->d < 1 || cs->d > WAVE)
+    return set_error("prh_cascade_bh: n must be in [2, 144] and d in [1, 64]");
+  if (cs->nb < 2 || cs->nb > PRH_BH_MAX) return set_error("prh_cascade_bh: nb must be in [2, 64]");
+  if (!cs->E || !cs->Ecr || !cs->eps_b || !cs->kern_b)
+    return set_error("prh_cascade_bh: null field-free table");
+  if (nwin < 0) return set_error("prh_cascade_bh: negative window count");
+  if (nwin == 0) return 0;
+  hipLaunchKernelGGL(cascade_bh_kernel, dim3((unsigned)nwin), dim3(256), 0, (hipStream_t)stream,
+                     *cs, nb_dens, rbh);
+  return check_hip(hipGetLastError(), "prh_cascade_bh launch");
+}
+
 int32_t prh_field_tables(const double* zs, const double* ss, int32_t nw, const double* P,
                          const double* W, const double* E0, int32_t ni, int32_t nm,
                          double planck_pref, double e_cmb, double zpow, double* out,
@@ -2470,9 +2979,22 @@ int32_t prh_uneven_step(const prh_operator* op, const prh_uneven* un, const prh_
                         int32_t step, void* stream) {
   if (int32_t e = check_uneven(op, un, st, "prh_uneven_step")) return e;
   if (step < 0 || step >= un->nsteps) return set_error("prh_uneven_step: step outside the schedule");
+  if (st->em) {
+    const prh_em* em = st->em;
+    if (em->d_em < 1 || em->d_em > PRH_EM_MAX)
+      return set_error("prh_uneven_step: d_em must be in [1, 144]");
+    if (!em->egrid || !em->dcoef || !em->ddiag || !em->cbh || !em->bhw || !em->rbh || !em->tg ||
+        !em->te || !em->gam)
+      return set_error("prh_uneven_step: EM sector with a null buffer");
+    if (em->nwin != un->nwin)
+      return set_error("prh_uneven_step: the EM tables need one entry per column window");
+  }
   hipStream_t s = (hipStream_t)stream;
   launch_uneven<false>(*op, *un, *st, step, s);
   launch_uneven<true>(*op, *un, *st, step, s);
+  if (st->em)
+    hipLaunchKernelGGL(em_uneven_kernel, dim3((unsigned)op->K), dim3(256), 0, s, *op, *un, *st,
+                       *st->em, step);
   return check_hip(hipGetLastError(), "prh_uneven_step launch");
 }
 

@@ -35,6 +35,7 @@ class PriNCeRun:
         photon_grid=None,
         backend=None,
         em_cascade=False,
+        kernel_cache=None,
     ):
         self.backend = backend if backend is not None else config.BackendConfig()
         self.cr_grid = EnergyGrid(*(cosmic_ray_grid or config.cosmic_ray_grid))
@@ -53,7 +54,7 @@ class PriNCeRun:
         self.pair_loss_rates_grid = ContinuousPairProductionLossRate(self, energy="grid")
         self.adia_loss_rates_bins = ContinuousAdiabaticLossRate(self, energy="bins")
         self.pair_loss_rates_bins = ContinuousPairProductionLossRate(self, energy="bins")
-        self.int_rates = PhotoNuclearInteractionRate(self)
+        self.int_rates = PhotoNuclearInteractionRate(self, kernel_cache=kernel_cache)
         self.em_int_rates = None
         self.spec_man.add_grid("ph", self.ph_grid.d)
         # co-evolved EM cascade (propagation.py:250-316): photons and e+- on

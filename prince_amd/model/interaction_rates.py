@@ -37,6 +37,8 @@ Loss rates: adiabatic kappa = H(z) cm2sec E (``interaction_rates.py:
 
 from __future__ import annotations
 
+import os
+
 import numpy as np
 from scipy.integrate import trapezoid
 from scipy.interpolate import InterpolatedUnivariateSpline, RegularGridInterpolator
@@ -104,7 +106,12 @@ class _ResponseDifferences:
 class PhotoNuclearInteractionRate:
     """Dense FP64 rate kernel ``_batch_matrix`` + fixed-pattern ``coupling_mat``."""
 
-    def __init__(self, prince_run):
+    def __init__(self, prince_run, kernel_cache=None):
+        """``kernel_cache``: path of an ``.npz`` kernel cache (``kernel_cache.py``);
+        loaded when it exists (and matches the run), written after the build
+        otherwise."""
+        from . import kernel_cache as kc
+
         self.prince_run = prince_run
         self.cross_sections = prince_run.cross_sections
         self.spec_man = prince_run.spec_man
@@ -112,8 +119,23 @@ class PhotoNuclearInteractionRate:
         self.e_cosmicray = prince_run.cr_grid
         self._ratemat_zcache = None
         self._check_common_log_step()
+        if kernel_cache is not None and os.path.exists(kernel_cache):
+            data = kc.load(prince_run, kernel_cache)
+            n = prince_run.dim_states
+            self.coupling_mat = csr_matrix(
+                (np.zeros(data["indices"].size), data["indices"], data["indptr"]), shape=(n, n))
+            self._batch_rows = data["batch_rows"]
+            self._batch_cols = data["batch_cols"]
+            self._batch_matrix = np.ascontiguousarray(data["batch_matrix"])
+            self._batch_vec = np.zeros(self._batch_rows.size)
+            self.sortidx = None
+            self.loaded_from_cache = True
+            return
         rows, cols, vals = self._assemble_channels()
         self._store_in_csr_order(rows, cols, vals)
+        self.loaded_from_cache = False
+        if kernel_cache is not None:
+            kc.save(self, kernel_cache)
 
     @property
     def photon_field(self):

@@ -751,21 +751,30 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
     def _em_setup(self, zs, K, dv):
         """Device buffers of the EM sector: per-window cascade transfers T(z) and
         Bethe-Heitler shapes at the refresh redshifts (propagation.py:461-618),
-        the per-window pair weights and the photon state."""
+        built on the device (``DeviceCascade``), the per-window pair weights and
+        the photon state."""
         from ..model.cascade import cascade_kernels
         from ..model.partial_diff import single_block
+
+        from .cascade_device import DeviceCascade
 
         torch = _torch()
         run = self.prince_run
         if self.enable_decay:
             raise NotImplementedError("explicit decay with the EM cascade")
+        if self._split_fields() is not None:
+            raise NotImplementedError("per-column photon-field scales with the EM cascade need "
+                                      "the uneven batch (per-column cascade transfers)")
         eg = run.em_grid
         if eg.d > _lib.EM_MAX:
             raise NotImplementedError(f"the EM kernels take at most {_lib.EM_MAX} bins")
         ck = cascade_kernels(eg.grid, run.cr_grid.grid)
         field = run.photon_field
-        Tg, Te = ck.transfer(zs, field, self._em_dz)
-        Rbh = ck.bh_shape(zs, field)
+        dc = getattr(run, "_prh_device_cascade", None)
+        if dc is None or dc.device != self.device:
+            dc = DeviceCascade(ck, self.device)
+            run._prh_device_cascade = dc
+        Tg, Te, Rbh, iters = dc.build(zs, field, self._em_dz)
         E_p = run.cr_grid.grid
         dE = np.gradient(E_p)
         pair = self.pair_loss_rates_grid
@@ -788,8 +797,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         b = {
             "egrid": dv(eg.grid, np.float64), "dcoef": dv(dcoef, np.float64),
             "ddiag": dv(ddiag, np.float64), "cbh": dv(cbh, np.float64),
-            "bhw": dv(bhw, np.float64), "rbh": dv(Rbh, np.float64),
-            "tg": dv(Tg, np.float64), "te": dv(Te, np.float64),
+            "bhw": dv(bhw, np.float64), "rbh": Rbh, "tg": Tg, "te": Te, "iters": iters,
             "gam": dv(gam0, np.float64), "gam0": dv(gam0, np.float64),
             "gam2": torch.zeros((eg.d, K), dtype=torch.float64, device=dev),
             "wbh": torch.zeros((run.cr_grid.d, K), dtype=torch.float64, device=dev),
@@ -800,7 +808,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             rbh=b["rbh"].data_ptr(), tg=b["tg"].data_ptr(), te=b["te"].data_ptr(),
             gam=b["gam"].data_ptr(), wbh=b["wbh"].data_ptr(), gam2=b["gam2"].data_ptr(),
         )
-        self.em_iterations = list(getattr(ck, "iterations", []))
+        self.em_iterations = iters.cpu().numpy().tolist()
         return em, b
 
     # ---------------------------------------------------------------- device run

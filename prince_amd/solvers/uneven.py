@@ -14,7 +14,15 @@ workgroups of finished columns leave at once.
 
 Per-window photon vectors and pair-loss rates of every column come from
 ``prh_field_tables`` (the CMB Planck spectrum evaluated on the device plus the
-z-separable EBL tabulated at z = 0).  The EM cascade is not part of this path.
+z-separable EBL tabulated at z = 0).
+
+With the EM cascade (``prince_run.enable_em_cascade``) every column also
+carries photons on the EM grid: the transfer T_gamma, T_electron and the
+Bethe-Heitler shapes of every column window are built on the device before
+the march (``DeviceCascade``: the column's field CMB + s_k EBL at the
+window's redshift), and after the nuclear stages of a step each live column
+runs its own EM step (``prh_uneven_step`` with ``prh_state.em``): the
+reference's step hook (``propagation.py:867-878``) per column.
 """
 
 from __future__ import annotations
@@ -81,8 +89,6 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
             raise ValueError("every z_max must lie above final_z")
         MultiRHSPropagationSolverETD2.__init__(self, float(z_max.max()), final_z, prince_run, K=K,
                                                **kwargs)
-        if self.enable_em_cascade:
-            raise NotImplementedError("the uneven batch does not carry the EM cascade")
         self.z_max = z_max
         self.ebl_scale = (np.ones(K) if ebl_scale is None
                           else np.asarray(ebl_scale, dtype=np.float64).reshape(-1))
@@ -193,6 +199,9 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
         zw = np.asarray(zw)
         sw = np.asarray(sw)
         phot, rpair = self._device_tables(zw, sw, dev)
+        # the EM sector's Bethe-Heitler weights use the pair rate whether or not
+        # the nuclear pair loss is switched on (as the reference's hook)
+        rpair_em = rpair.clone() if self.enable_em_cascade else None
         if not self.enable_pairprod_losses:
             rpair.zero_()
         # operator pattern from the first window of the first column; every
@@ -257,16 +266,68 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
         )
         st = _lib.PrhState(x=x.data_ptr(), a=bufs["a"].data_ptr(), fphi=bufs["fphi"].data_ptr(),
                            step_ctl=bufs["step_ctl"].data_ptr())
+        if self.enable_em_cascade:
+            em_struct, em_bufs = self._em_tables(zw, sw, rpair_em, order, dz, dev, dv)
+            st.em = ctypes.addressof(em_struct)
+        else:
+            em_struct = em_bufs = None
         self._state_wide = False
         self._plan = {
             "op": op, "un": un, "st": st, "x": x, "bufs": bufs, "nsteps": T, "order": order,
             "ref_off": np.asarray(ref_off), "stride": stride, "dph": cache.dph, "K": K,
             "stream": torch.cuda.current_stream(dev).cuda_stream, "layout": wl, "kt": kt,
-            "nwin": int(zw.size), "col_steps": nsteps, "graph": None, "em_bufs": None,
-            "two": False, "wide": False,
+            "nwin": int(zw.size), "col_steps": nsteps, "graph": None, "em_bufs": em_bufs,
+            "em": em_struct, "two": False, "wide": False,
         }
         self.timings["setup_host_s"] = time.perf_counter() - t0
         return self._plan
+
+    def _em_tables(self, zw, sw, rpair, order, dz, dev, dv):
+        """Device EM tables of every column window (in device column order):
+        T_gamma, T_electron, R_BH, Bethe-Heitler weights; photon state."""
+        from ..model.cascade import cascade_kernels
+        from ..model.partial_diff import single_block
+        from .cascade_device import DeviceCascade
+        from .layout import fd_tables
+        from .run_inputs import pair_species_scale
+
+        torch = _torch()
+        run = self.prince_run
+        eg = run.em_grid
+        if eg.d > _lib.EM_MAX:
+            raise NotImplementedError(f"the EM kernels take at most {_lib.EM_MAX} bins")
+        ck = cascade_kernels(eg.grid, run.cr_grid.grid)
+        cache = getattr(run, "_prh_device_cascade", None)
+        if cache is None or cache.device != dev:
+            cache = DeviceCascade(ck, dev)
+            run._prh_device_cascade = cache
+        t0 = time.perf_counter()
+        tg, te, rbh, iters = cache.build(zw, run.photon_field, dz, scales=sw)
+        E_p = run.cr_grid.grid
+        dl = PRINCE_UNITS.c * abs(dz) / ((1.0 + zw) * np.array([H(z) for z in zw]))
+        dEoE = np.gradient(E_p) / E_p
+        bhw = (torch.from_numpy(dl).to(dev)[:, None] * torch.from_numpy(dEoE).to(dev)[None, :]) * rpair
+        pscale = pair_species_scale(self.pair_loss_rates_grid, self.spec_man)
+        cbh = np.zeros(self.spec_man.nspec)
+        for sref in self.spec_man.species_refs:
+            if sref.is_nucleus and abs(sref.charge) >= 1:
+                cbh[sref.princeidx] = float(sref.A) * pscale[sref.princeidx]
+        dcoef, ddiag = fd_tables(single_block(eg))
+        b = {
+            "egrid": dv(eg.grid, np.float64), "dcoef": dv(dcoef, np.float64),
+            "ddiag": dv(ddiag, np.float64), "cbh": dv(cbh, np.float64),
+            "bhw": bhw.contiguous(), "rbh": rbh, "tg": tg, "te": te, "iters": iters,
+            "gam": torch.zeros((eg.d, self._K), dtype=torch.float64, device=dev),
+        }
+        em = _lib.PrhEm(
+            d_em=eg.d, nwin=int(zw.size), egrid=b["egrid"].data_ptr(), dcoef=b["dcoef"].data_ptr(),
+            ddiag=b["ddiag"].data_ptr(), cbh=b["cbh"].data_ptr(), bhw=b["bhw"].data_ptr(),
+            rbh=b["rbh"].data_ptr(), tg=b["tg"].data_ptr(), te=b["te"].data_ptr(),
+            gam=b["gam"].data_ptr(), wbh=None, gam2=None,
+        )
+        torch.cuda.synchronize(dev)
+        self.timings["em_build_s"] = time.perf_counter() - t0
+        return em, b
 
     def _chunk_graphs(self, plan):
         """Capture the solve as 64-step chunk graphs (re-folds and stages of every
@@ -307,6 +368,8 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
         k_end = plan["nsteps"] if k_end is None else k_end
         if k_begin == 0:
             plan["x"].zero_()
+            if plan.get("em_bufs") is not None:
+                plan["em_bufs"]["gam"].zero_()
         C = self.graph_chunk
         if (getattr(self.backend, "use_hip_graphs", True) and k_begin % C == 0
                 and (k_end % C == 0 or k_end == plan["nsteps"])):
@@ -335,6 +398,10 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
         torch.cuda.synchronize(self.device)
         nspec, d, K = self.spec_man.nspec, self.prince_run.cr_grid.d, self._K
         host = plan["x"].cpu().numpy().transpose(0, 2, 1).reshape(nspec * d, K)
+        if plan.get("em_bufs") is not None:
+            gam = plan["em_bufs"]["gam"].cpu().numpy()
+            z = np.zeros_like(gam)
+            host = np.concatenate([host, z, z, gam], axis=0)  # e+, e-, photon blocks
         state = np.empty_like(host)
         state[:, plan["order"]] = host
         self.state = state

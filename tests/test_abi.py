@@ -35,7 +35,7 @@ def test_library_exports_declared_symbols():
 
 def test_abi_rejects_bad_arguments_without_touching_the_gpu():
     lib = _lib.load()
-    assert lib.prh_fold_gemm(None, 10, 200, None, 1, None, 10, None) != 0
+    assert lib.prh_fold_gemm(None, 10, 5000, None, 1, None, 10, None) != 0
     assert b"dph" in lib.prh_last_error()
     assert lib.prh_gather_rows(None, None, -1, 72, None, None) != 0
     assert lib.prh_etd2_steps(None, None, None, 0, 1, None) != 0

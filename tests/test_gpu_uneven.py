@@ -78,3 +78,41 @@ def test_uneven_chunk_graphs_equal_direct_launches():
         states.append(g.state.copy())
     run.backend.use_hip_graphs = True
     assert np.array_equal(states[0], states[1])
+
+
+def test_uneven_batch_with_em_cascade_matches_column_solves():
+    """The uneven batch with the co-evolved EM cascade: every column (own
+    z_max, own EBL scale, hence own windows and own device-built cascade
+    transfer) against the oracle solve of that column on the field
+    CMB + s_k EBL -- nuclear states and photon spectrum."""
+    run = workloads.nuclear_em_run()
+    K = 4
+    rng = np.random.default_rng(13)
+    z_max = rng.uniform(0.1, 0.16, size=K)
+    scale = rng.uniform(0.6, 1.4, size=K)
+    srcs = workloads.batch_sources(run, K, seed=9)
+    g = UnevenBatchSolverETD2(z_max, run, ebl_scale=scale)
+    for k in range(K):
+        g[k].add_source_class(srcs[k])
+    g.solve(dz=1e-3)
+    n = run.dim_states
+    sl = run.em_slices()
+    assert g.state.shape == (n + run.dim_em, K)
+    base = run.photon_field
+    worst_n = worst_g = 0.0
+    try:
+        for k in range(K):
+            run.photon_field = column_field(base, scale[k])
+            o = OracleETD2(float(z_max[k]), 0.0, run)
+            o.add_source_class(srcs[k])
+            o.solve(1e-3)
+            dn, nbn = rel_dev(g.state[:n, k], o.state)
+            dg, nbg = rel_dev(g.state[sl[22], k], o.em_state[22])
+            print(f"column {k}: z_max {z_max[k]:.4f} scale {scale[k]:.3f} nuclear {dn:.3e} "
+                  f"({nbn} bins) photons {dg:.3e} ({nbg} bins)")
+            assert nbn > 100 and nbg > 10
+            worst_n, worst_g = max(worst_n, dn), max(worst_g, dg)
+    finally:
+        run.photon_field = base
+    assert not np.any(g.state[sl[11]]) and not np.any(g.state[sl[-11]])
+    assert worst_n <= TOL and worst_g <= TOL
