@@ -352,6 +352,33 @@ int32_t prh_graph_run(prh_graph* g, int32_t k_begin, int32_t k_end, void* stream
 int32_t prh_graph_destroy(prh_graph* g);
 
 /* ---------------------------------------------------------------------------
+ * Rate kernel rebuilt on the device from its Toeplitz factors.
+ * ------------------------------------------------------------------------- */
+typedef struct prh_toeplitz {
+  int32_t d;            /* cosmic-ray bins */
+  int32_t dph;          /* photon bins */
+  int32_t n1;           /* 1-D tables (inclusive and non-elastic response differences) */
+  int32_t n2;           /* 2-D tables (redistributed channels) */
+  int64_t nrows;        /* kernel rows (descriptors), in the CSR order of the coupling matrix */
+  const int32_t* desc;  /* [nrows][5]: kind (0 boost-conserving, 1 redistributed), t_gain, t_loss, i, k */
+  const double* tab1d;  /* [n1][d + dph - 1] */
+  const double* tab2d;  /* [n2][d + dph - 1][2 d - 1] */
+  const double* fmo;    /* [d] m_p / E_i */
+  const double* fdiff;  /* [d][d] (m_p / E_i) dE_i / dE_k */
+} prh_toeplitz;
+
+/* Kernel rows sel[r] (-1: a zero row) expanded into out [nsel][dph]: row s
+ * with descriptor (kind, tg, tl, i, k) is, at photon bin j (w = i + j),
+ *   kind 0: fmo[i] tab1d[tg][w] - fmo[i] tab1d[tl][w]
+ *   kind 1: max(fdiff[i][k] tab2d[tg][w][k - i + d - 1], 0) - [k == i] fmo[i] tab1d[tl][w]
+ * (a term with table index -1 absent) -- the tile construction of
+ * interaction_rates.py:488-595 for boost-conserving and redistributed
+ * channels, evaluated on the device from the per-channel tables instead of
+ * uploading the dense kernel (the reference's _batch_matrix). */
+int32_t prh_toeplitz_rows(const prh_toeplitz* tz, const int64_t* sel, int64_t nsel, double* out,
+                          void* stream);
+
+/* ---------------------------------------------------------------------------
  * EM-cascade window operators built on the device.
  * ------------------------------------------------------------------------- */
 #define PRH_BH_MAX 64 /* target-photon points of the Bethe-Heitler kernel */

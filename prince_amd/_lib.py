@@ -33,6 +33,7 @@ EXPORTED = (
     "prh_uneven_graph_create",
     "prh_cascade_transfer",
     "prh_cascade_bh",
+    "prh_toeplitz_rows",
 )
 
 WG_WAVES = 16
@@ -198,6 +199,21 @@ class HipLibraryError(RuntimeError):
     pass
 
 
+class PrhToeplitz(ctypes.Structure):
+    _fields_ = [
+        ("d", _i32),
+        ("dph", _i32),
+        ("n1", _i32),
+        ("n2", _i32),
+        ("nrows", _i64),
+        ("desc", _p),
+        ("tab1d", _p),
+        ("tab2d", _p),
+        ("fmo", _p),
+        ("fdiff", _p),
+    ]
+
+
 class PrhCascade(ctypes.Structure):
     _fields_ = [
         ("n", _i32),
@@ -262,6 +278,8 @@ def load():
     lib.prh_cascade_transfer.restype = _i32
     lib.prh_cascade_transfer.argtypes = [ctypes.POINTER(PrhCascade), _p, _i64, _p, _p, _i32, _p,
                                          _i64, _p, _p, _p, _p]
+    lib.prh_toeplitz_rows.restype = _i32
+    lib.prh_toeplitz_rows.argtypes = [ctypes.POINTER(PrhToeplitz), _p, _i64, _p, _p]
     lib.prh_cascade_bh.restype = _i32
     lib.prh_cascade_bh.argtypes = [ctypes.POINTER(PrhCascade), _p, _i32, _p, _p]
     lib.prh_etd2_stage.restype = _i32

@@ -151,6 +151,38 @@ __global__ void __launch_bounds__(THREADS) contract_rows_mfma(const double* __re
   }
 }
 
+// Rows of the rate kernel expanded from their Toeplitz descriptors (one
+// thread per output value; the same products in the same order as the host
+// construction, so the rows equal the dense kernel's bit for bit).
+__global__ void toeplitz_rows_kernel(prh_toeplitz tz, const int64_t* __restrict__ sel,
+                                     int64_t nsel, double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int dph = tz.dph;
+  if (t >= nsel * dph) return;
+  const int64_t r = t / dph;
+  const int j = (int)(t - r * dph);
+  const int64_t s = sel[r];
+  if (s < 0 || s >= tz.nrows) {
+    out[t] = 0.0;
+    return;
+  }
+  const int32_t* ds = tz.desc + 5 * s;
+  const int kind = ds[0], tg = ds[1], tl = ds[2], i = ds[3], k = ds[4];
+  const int d = tz.d;
+  const int64_t L = d + dph - 1;
+  const int w = i + j;
+  const double fm = tz.fmo[i];
+  double base;
+  if (kind == 0) {
+    base = tg >= 0 ? fm * tz.tab1d[tg * L + w] : 0.0;
+  } else {
+    const double v = tz.fdiff[i * d + k] * tz.tab2d[(tg * L + w) * (2 * d - 1) + (k - i + d - 1)];
+    base = v >= 0.0 ? v : 0.0;
+  }
+  const bool loss = tl >= 0 && (kind == 0 || i == k);
+  out[t] = loss ? base - fm * tz.tab1d[tl * L + w] : base;
+}
+
 __global__ void gather_rows_kernel(const double* __restrict__ src, const int64_t* __restrict__ sel,
                                    int64_t nrows, int ncols, double* __restrict__ dst) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2347,13 +2379,17 @@ __global__ void __launch_bounds__(CTHREADS) cascade_transfer_kernel(
     sE[i] = i < n ? cs.E[i] : 0.0;
     sdE[i] = i < n ? cs.dE[i] : 0.0;
   }
-  // emission into E1, pair spectrum into P, zero the rest (padding included)
+  // emission into E1, pair spectrum into P, every other matrix zeroed: the
+  // padded rows and columns (n <= i, j < 144) take part in the tile products
+  // and must hold zeros, never stale workspace contents
   for (int64_t t = tid; t < CMAT; t += CTHREADS) {
     const int i = (int)(t / CNP), j = (int)(t % CNP);
     const bool in = i < n && j < n;
     E1[t] = in ? cs.pref_emis[j] * cw[crow_emis(n) + (int64_t)i * n + j] : 0.0;
     P[t] = in ? cs.pref_pair[j] * cw[crow_pair(n) + (int64_t)i * n + j] : 0.0;
+    Sg[t] = 0.0;
     De[t] = 0.0;
+    Mic[t] = 0.0;
     G[t] = (in && i == j) ? 1.0 : 0.0;
     Ev[t] = 0.0;
     Esc[t] = 0.0;
@@ -2863,6 +2899,23 @@ int32_t prh_graph_destroy(prh_graph* g) {
   return 0;
 }
 
+
+int32_t prh_toeplitz_rows(const prh_toeplitz* tz, const int64_t* sel, int64_t nsel, double* out,
+                          void* stream) {
+  if (!tz || !sel || !out) return set_error("prh_toeplitz_rows: null argument");
+  if (tz->d < 1 || tz->dph < 1 || tz->n1 < 1 || tz->n2 < 1 || tz->nrows < 0)
+    return set_error("prh_toeplitz_rows: empty factor tables");
+  if (!tz->desc || !tz->tab1d || !tz->tab2d || !tz->fmo || !tz->fdiff)
+    return set_error("prh_toeplitz_rows: null factor table");
+  if (nsel < 0) return set_error("prh_toeplitz_rows: negative row count");
+  if (nsel == 0) return 0;
+  const int64_t total = nsel * tz->dph;
+  const int64_t blocks = (total + 255) / 256;
+  if (blocks > 0x7fffffff) return set_error("prh_toeplitz_rows: too many rows");
+  hipLaunchKernelGGL(toeplitz_rows_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                     (hipStream_t)stream, *tz, sel, nsel, out);
+  return check_hip(hipGetLastError(), "prh_toeplitz_rows launch");
+}
 
 int32_t prh_cascade_transfer(const prh_cascade* cs, const double* contr, int64_t cstride,
                              const double* dtau, const double* dl, int32_t nwin, double* work,

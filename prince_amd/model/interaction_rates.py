@@ -103,6 +103,94 @@ class _ResponseDifferences:
         return {key: diffs[row] for row, key in enumerate(keys)}
 
 
+class ToeplitzFactors:
+    """The rate kernel in factored form: per channel one table of response
+    antiderivative differences, per kernel row a descriptor.
+
+    Row r of ``_batch_matrix`` (CSR order) with descriptor
+    ``desc[r] = (kind, t_gain, t_loss, i, k)`` is, for photon bin j,
+
+    * kind BC:     fmo[i] tab1d[t_gain][i + j] - fmo[i] tab1d[t_loss][i + j]
+    * kind REDIST: max(fdiff[i][k] tab2d[t_gain][i + j][k - i + d - 1], 0)
+                   - fmo[i] tab1d[t_loss][i + j]
+
+    where a term with table index -1 is absent (a BC row without gain is
+    0 - fmo tab1d[t_loss]; the REDIST loss term only on the diagonal k = i of
+    a self channel), fmo[i] = m_p / E_i and fdiff[i][k] = fmo[i] dE_i / dE_k.
+    The same products in the same order as the dense construction, so the
+    expansion reproduces ``_batch_matrix`` bit for bit.  Storage: the 1-D
+    tables (d + dph - 1 values) and the 2-D tables ((d + dph - 1) x (2d - 1)
+    values, redistributed channels only) instead of rows x dph values.
+    """
+
+    BC, REDIST = 0, 1
+
+    def __init__(self, d, dph, fmo, fdiff):
+        self.d, self.dph = int(d), int(dph)
+        self.fmo = np.ascontiguousarray(fmo, dtype=np.float64)
+        self.fdiff = np.ascontiguousarray(fdiff, dtype=np.float64)
+        self._t1, self._t2 = [], []
+        self._idx1, self._idx2 = {}, {}
+        self.desc = None
+
+    def table_1d(self, key, vec):
+        if vec is None:
+            return -1
+        if key not in self._idx1:
+            self._idx1[key] = len(self._t1)
+            self._t1.append(np.asarray(vec, dtype=np.float64))
+        return self._idx1[key]
+
+    def table_2d(self, key, mat):
+        if key not in self._idx2:
+            self._idx2[key] = len(self._t2)
+            self._t2.append(np.asarray(mat, dtype=np.float64))
+        return self._idx2[key]
+
+    @staticmethod
+    def rows(kind, t_gain, t_loss, i, k):
+        i = np.asarray(i, dtype=np.int32)
+        out = np.empty((i.size, 5), dtype=np.int32)
+        out[:, 0] = kind
+        out[:, 1] = t_gain
+        out[:, 2] = t_loss
+        out[:, 3] = i
+        out[:, 4] = np.asarray(k, dtype=np.int32)
+        return out
+
+    def finish(self):
+        L = self.d + self.dph - 1
+        self.tab1d = np.ascontiguousarray(np.stack(self._t1) if self._t1 else np.zeros((1, L)))
+        self.tab2d = np.ascontiguousarray(
+            np.stack(self._t2) if self._t2 else np.zeros((1, L, 2 * self.d - 1)))
+        del self._t1, self._t2
+
+    @property
+    def nbytes(self):
+        return self.tab1d.nbytes + self.tab2d.nbytes + self.desc.nbytes + self.fdiff.nbytes
+
+    def expand(self, rows=None):
+        """Host expansion of the listed kernel rows (all by default) [nrows, dph]."""
+        desc = self.desc if rows is None else self.desc[np.asarray(rows)]
+        d, dph = self.d, self.dph
+        j = np.arange(dph)[None, :]
+        kind, tg, tl, i, k = (desc[:, c] for c in range(5))
+        out = np.zeros((desc.shape[0], dph))
+        w = i[:, None] + j
+        bc = kind == self.BC
+        fm = self.fmo[i][:, None]
+        t1 = np.where(bc & (tg >= 0), tg, 0)[:, None]
+        t2 = np.where(bc, 0, tg)[:, None]
+        gain_bc = np.where((tg >= 0)[:, None], fm * self.tab1d[t1, w], 0.0)
+        red = self.fdiff[i, k][:, None] * self.tab2d[t2, w, (k - i + d - 1)[:, None]]
+        red = np.maximum(red, 0.0)
+        base = np.where(bc[:, None], gain_bc, red)
+        has_loss = (tl >= 0) & (bc | (i == k))
+        loss = fm * self.tab1d[np.maximum(tl, 0)[:, None], w]
+        out[:] = np.where(has_loss[:, None], base - loss, base)
+        return out
+
+
 class PhotoNuclearInteractionRate:
     """Dense FP64 rate kernel ``_batch_matrix`` + fixed-pattern ``coupling_mat``."""
 
@@ -129,6 +217,7 @@ class PhotoNuclearInteractionRate:
             self._batch_matrix = np.ascontiguousarray(data["batch_matrix"])
             self._batch_vec = np.zeros(self._batch_rows.size)
             self.sortidx = None
+            self.toeplitz = None
             self.loaded_from_cache = True
             return
         rows, cols, vals = self._assemble_channels()
@@ -204,10 +293,12 @@ class PhotoNuclearInteractionRate:
         mo_bin, da_bin = np.meshgrid(ibin, ibin, indexing="ij")
 
         sref = self.spec_man.pdgid2sref
-        rows, cols, vals = [], [], []
+        fac = ToeplitzFactors(d, dph, mother_scale, redist_scale)
+        rows, cols, vals, descs = [], [], [], []
         for mo, da, kind in self._channels():
             r0, c0 = sref[da].lidx(), sref[mo].lidx()
             loss = tabs.nonel.get(mo) if mo == da else None
+            t_loss = fac.table_1d(("nonel", mo), loss)
             if kind == "bc":
                 gain = tabs.incl.get((mo, da))
                 if gain is None and loss is None:
@@ -218,6 +309,8 @@ class PhotoNuclearInteractionRate:
                 vals.append(np.broadcast_to(tile, (d, dph)).copy())
                 rows.append(r0 + ibin)
                 cols.append(c0 + ibin)
+                descs.append(fac.rows(ToeplitzFactors.BC, fac.table_1d(("incl", mo, da), gain),
+                                      t_loss, ibin, ibin))
                 continue
             cube = redist_scale[:, :, None] * tabs.redist[(mo, da)][window[:, None, :], x_col[:, :, None]]
             np.maximum(cube, 0.0, out=cube)
@@ -230,6 +323,10 @@ class PhotoNuclearInteractionRate:
             vals.append(cube[keep])
             rows.append(r0 + da_bin[keep])
             cols.append(c0 + mo_bin[keep])
+            descs.append(fac.rows(ToeplitzFactors.REDIST, fac.table_2d((mo, da), tabs.redist[(mo, da)]),
+                                  t_loss, mo_bin[keep], da_bin[keep]))
+        self._toeplitz_unsorted = fac
+        fac.desc = np.concatenate(descs, axis=0)
         return (np.concatenate(rows).astype(np.int64), np.concatenate(cols).astype(np.int64),
                 np.concatenate(vals, axis=0))
 
@@ -247,6 +344,11 @@ class PhotoNuclearInteractionRate:
         self._batch_cols = cols
         self._batch_matrix = np.ascontiguousarray(vals[order, :])
         self._batch_vec = np.zeros(rows.size)
+        fac = self.__dict__.pop("_toeplitz_unsorted", None)
+        if fac is not None:
+            fac.desc = np.ascontiguousarray(fac.desc[order])
+            fac.finish()
+        self.toeplitz = fac
 
     # ------------------------------------------------------------ evaluation
     def get_hadr_jacobian(self, z, scale_fac=1.0, force_update=False, pfield=None):

@@ -222,14 +222,62 @@ class UHECRPropagationSolverETD2(UHECRPropagationSolver):
 
 
 class _HipDeviceCache:
-    """Per-run device residency: the dense fold kernel B (uploaded once)."""
+    """Per-run device residency of the rate kernel.
+
+    When the run's rate object carries the kernel's Toeplitz factors
+    (``int_rates.toeplitz``: per-channel response tables and one descriptor
+    per kernel row, a quarter of the dense kernel's bytes for the A <= 56 run)
+    only those are uploaded and every kernel row the solver needs is expanded
+    on the device (``prh_toeplitz_rows``); otherwise the dense kernel
+    ``_batch_matrix`` (the reference's object) is uploaded once and rows are
+    gathered from it (``prh_gather_rows``).  Both give the same rows bit for
+    bit.
+    """
 
     def __init__(self, run, device):
         torch = _torch()
-        B = fold_kernel_fp64(run.int_rates)
-        self.B_full = torch.from_numpy(B).to(device)
-        self.nrows, self.dph = B.shape
+        self.device = device
         self.layouts = {}
+        self.lib = _lib.load()
+        ir = run.int_rates
+        tz = getattr(ir, "toeplitz", None)
+        self.tz = None
+        self.B_full = None
+        if tz is not None:
+            desc = np.ascontiguousarray(tz.desc, dtype=np.int32)
+            kind, tg, tl, i, k = (desc[:, c] for c in range(5))
+            n1, n2 = tz.tab1d.shape[0], tz.tab2d.shape[0]
+            ok = (np.all((kind == 0) | (kind == 1)) and np.all(tl < n1)
+                  and np.all(np.where(kind == 0, tg < n1, (tg >= 0) & (tg < n2)))
+                  and np.all((i >= 0) & (i < tz.d) & (k >= 0) & (k < tz.d)))
+            if not ok or desc.shape[0] != ir.coupling_mat.nnz:
+                raise ValueError("Toeplitz factors of the rate kernel are inconsistent")
+            dv = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+            self.tz_bufs = {"desc": dv(desc), "tab1d": dv(tz.tab1d), "tab2d": dv(tz.tab2d),
+                            "fmo": dv(tz.fmo), "fdiff": dv(tz.fdiff)}
+            b = self.tz_bufs
+            self.tz = _lib.PrhToeplitz(
+                d=tz.d, dph=tz.dph, n1=n1, n2=n2, nrows=desc.shape[0],
+                desc=b["desc"].data_ptr(), tab1d=b["tab1d"].data_ptr(),
+                tab2d=b["tab2d"].data_ptr(), fmo=b["fmo"].data_ptr(), fdiff=b["fdiff"].data_ptr())
+            self.nrows, self.dph = desc.shape[0], tz.dph
+            self.resident_bytes = int(sum(t.numel() * t.element_size() for t in b.values()))
+        else:
+            B = fold_kernel_fp64(ir)
+            self.B_full = torch.from_numpy(B).to(device)
+            self.nrows, self.dph = B.shape
+            self.resident_bytes = B.nbytes
+
+    def rows(self, sel_dev, nsel, out, stream):
+        """Kernel rows sel (int64 device tensor, -1: zero row) into out [nsel][dph]."""
+        import ctypes as _ct
+
+        if self.tz is not None:
+            _lib.check(self.lib.prh_toeplitz_rows(_ct.byref(self.tz), sel_dev.data_ptr(), nsel,
+                                                  out.data_ptr(), stream), "kernel rows (Toeplitz)")
+        else:
+            _lib.check(self.lib.prh_gather_rows(self.B_full.data_ptr(), sel_dev.data_ptr(), nsel,
+                                                self.dph, out.data_ptr(), stream), "kernel row gather")
 
 
 class ETD2SolverHIP(UHECRPropagationSolverETD2):
@@ -405,7 +453,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
     def _device_cache(self):
         run = self.prince_run
         cache = getattr(run, "_prh_device_cache", None)
-        if cache is None or cache.B_full.device != self.device:
+        if cache is None or cache.device != self.device:
             cache = _HipDeviceCache(run, self.device)
             run._prh_device_cache = cache
         return cache
@@ -425,14 +473,20 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         stream = torch.cuda.current_stream(self.device).cuda_stream
         ph = torch.from_numpy(np.ascontiguousarray(phot0[None, :])).to(self.device)
         m0 = torch.empty(cache.nrows, dtype=torch.float64, device=self.device)
+        full = cache.B_full
+        if full is None:  # every row expanded once, for the first-window fold only
+            full = torch.empty((cache.nrows, cache.dph), dtype=torch.float64, device=self.device)
+            ident = torch.arange(cache.nrows, dtype=torch.int64, device=self.device)
+            cache.rows(ident, cache.nrows, full, stream)
         _lib.check(
             self._lib.prh_fold_gemm(
-                cache.B_full.data_ptr(), cache.nrows, cache.dph, ph.data_ptr(), 1,
+                full.data_ptr(), cache.nrows, cache.dph, ph.data_ptr(), 1,
                 m0.data_ptr(), cache.nrows, stream,
             ),
             "first-window fold",
         )
         m0_host = m0.cpu().numpy()
+        del full
         ir = self.had_int_rates
         args = (ir.coupling_mat.indptr, ir.coupling_mat.indices, m0_host, self.spec_man.nspec,
                 self.prince_run.cr_grid.d, bool(self.enable_photohad_losses))
@@ -442,13 +496,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         nsel = wl.sel.size
         sel_dev = torch.from_numpy(wl.sel).to(self.device)
         B_sel = torch.empty((nsel, cache.dph), dtype=torch.float64, device=self.device)
-        _lib.check(
-            self._lib.prh_gather_rows(
-                cache.B_full.data_ptr(), sel_dev.data_ptr(), nsel, cache.dph,
-                B_sel.data_ptr(), stream,
-            ),
-            "split fold kernel gather",
-        )
+        cache.rows(sel_dev, nsel, B_sel, stream)
         lay = {"layout": wl, "B_sel": B_sel, "m0": m0_host}
         cache.layouts[key] = lay
         return lay

@@ -107,3 +107,19 @@ def test_redistributed_tiles_against_cell_quadrature(mo):
     if nonel is not None:
         assert neg_diag > 10
     print(f"mother {mo}: {kept} kept rows match the cell quadrature ({clipped} negative cells)")
+
+
+@pytest.mark.parametrize("which", ["proton", "nuclear"])
+def test_toeplitz_factors_expand_to_the_dense_kernel(which):
+    """The factored kernel (per-channel tables + row descriptors) expands to
+    the dense kernel bit for bit, in a quarter of its bytes or less."""
+    run = workloads.proton_run() if which == "proton" else workloads.nuclear_run()
+    ir = run.int_rates
+    tz = ir.toeplitz
+    assert tz.desc.shape == (ir._batch_matrix.shape[0], 5)
+    full = tz.expand()
+    assert np.array_equal(full.view(np.int64), ir._batch_matrix.view(np.int64))
+    rows = np.array([0, 7, full.shape[0] - 1])
+    assert np.array_equal(tz.expand(rows), ir._batch_matrix[rows])
+    if which == "nuclear":
+        assert tz.nbytes < 0.26 * ir._batch_matrix.nbytes

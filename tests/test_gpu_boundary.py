@@ -43,3 +43,31 @@ def test_reference_view_float32_csr_kernel_against_oracle():
     print(f"float32 CSR kernel: vs oracle {dev:.3e} ({nb} bins); vs float64 kernel {d32:.3e}")
     assert nb > 1000 and dev <= 1e-8
     assert d32 > 0.0  # the float32 kernel is a different operator
+
+
+def test_toeplitz_rows_on_device_equal_dense_gather():
+    """Kernel rows expanded on the device from the Toeplitz factors equal the
+    rows gathered from the dense kernel, bit for bit (block-band selection of
+    the first window, with the transposed slots of an adjoint plan)."""
+    import torch
+
+    from prince_amd.solvers.propagation import _HipDeviceCache
+
+    run = workloads.nuclear_run()
+    dev = torch.device("cuda", 0)
+    fac = _HipDeviceCache(run, dev)
+    dense = _HipDeviceCache(reference_view(run, "dense"), dev)
+    assert fac.tz is not None and dense.tz is None
+    assert fac.resident_bytes < 0.3 * dense.resident_bytes
+    rng = np.random.default_rng(2)
+    sel = rng.integers(-1, fac.nrows, size=20000)
+    sel[:5] = [0, fac.nrows - 1, -1, 17, 123456]
+    st = torch.from_numpy(sel).to(dev)
+    a = torch.empty((sel.size, fac.dph), dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    fac.rows(st, sel.size, a, stream)
+    dense.rows(st, sel.size, b, stream)
+    torch.cuda.synchronize()
+    assert torch.equal(a.view(torch.int64), b.view(torch.int64))
+    assert not torch.any(a[2])
