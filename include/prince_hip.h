@@ -149,11 +149,23 @@ typedef struct prh_em {
   const double* rbh;    /* [nwin][d_em][d] Bethe-Heitler e+- shape (one sign) */
   const double* tg;     /* [nwin][d_em][d_em] photon transfer T_gamma */
   const double* te;     /* [nwin][d_em][d_em] e+- transfer T_electron */
-  double* gam;          /* [d_em][K] photon state read by even steps, written by odd ones */
+  double* gam;          /* [d_em][K] photon state, updated in place every step */
   double* wbh;          /* [d][K] scratch: Bethe-Heitler weights of the step */
-  double* gam2;         /* [d_em][K] photon state written by even steps, read by odd ones
-                         * (after step s the state is in gam2 if s is even, else gam) */
+  double* gam2;         /* [d_em][K] scratch: photons after the ETD2 step, before the transfer */
+  /* [nwin][d_em][d_em + d] per window [T_gamma | 2 T_electron R_BH]
+   * (prh_em_transfer_stack): the step's transfer is one product with
+   * [gam2; wbh].  Shared-schedule steps need it; the uneven batch uses tg, te
+   * and rbh directly. */
+  const double* tstack;
 } prh_em;
+
+/* Per window w: out[w] = [T_gamma[w] | 2 T_electron[w] R_BH[w]] ([d_em][d_em + d]),
+ * the stacked transfer of the shared-schedule EM step: e+ = e- = R_BH w_BH
+ * feed T_electron (e- + e+), so T_electron 2 R_BH acts on the Bethe-Heitler
+ * weights directly (the reference's _inject_bh_pairs followed by
+ * _apply_em_cascade, propagation.py:555-571, 620-665, as one product). */
+int32_t prh_em_transfer_stack(const double* tg, const double* te, const double* rbh,
+                              int32_t nwin, int32_t d_em, int32_t d, double* out, void* stream);
 
 /* Opaque handle of a captured step graph (prh_graph_create, prh_uneven_graph_create). */
 typedef struct prh_graph prh_graph;

@@ -34,6 +34,7 @@ EXPORTED = (
     "prh_cascade_transfer",
     "prh_cascade_bh",
     "prh_toeplitz_rows",
+    "prh_em_transfer_stack",
 )
 
 WG_WAVES = 16
@@ -139,6 +140,7 @@ class PrhEm(ctypes.Structure):
         ("gam", _p),
         ("wbh", _p),
         ("gam2", _p),
+        ("tstack", _p),
     ]
 
 
@@ -280,6 +282,8 @@ def load():
                                          _i64, _p, _p, _p, _p]
     lib.prh_toeplitz_rows.restype = _i32
     lib.prh_toeplitz_rows.argtypes = [ctypes.POINTER(PrhToeplitz), _p, _i64, _p, _p]
+    lib.prh_em_transfer_stack.restype = _i32
+    lib.prh_em_transfer_stack.argtypes = [_p, _p, _p, _i32, _i32, _i32, _p, _p]
     lib.prh_cascade_bh.restype = _i32
     lib.prh_cascade_bh.argtypes = [ctypes.POINTER(PrhCascade), _p, _i32, _p, _p]
     lib.prh_etd2_stage.restype = _i32

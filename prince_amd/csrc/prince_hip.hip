@@ -156,6 +156,9 @@ __global__ void __launch_bounds__(THREADS) contract_rows_mfma(const double* __re
 // construction, so the rows equal the dense kernel's bit for bit).
 __global__ void toeplitz_rows_kernel(prh_toeplitz tz, const int64_t* __restrict__ sel,
                                      int64_t nsel, double* __restrict__ out) {
+  // every product rounded on its own, as the host construction does (no
+  // fused multiply-add: fm * gain - fm * loss must round twice)
+#pragma clang fp contract(off)
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int dph = tz.dph;
   if (t >= nsel * dph) return;
@@ -1350,10 +1353,6 @@ void launch_wide(const prh_operator& op, const prh_schedule& sch, const prh_stat
 // em_step: one 1024-thread workgroup per 64-rhs group (lane = rhs); the photon
 // column lives in LDS ([bin][lane], conflict-free); wave v owns bins v + 16 t.
 // ---------------------------------------------------------------------------
-constexpr int EM_WAVES = 16;
-constexpr int EM_THREADS = EM_WAVES * WAVE;
-constexpr int EM_NB = (PRH_EM_MAX + EM_WAVES - 1) / EM_WAVES;
-constexpr int EM_RC = 17;  // workgroups (row chunks of the transfer) per 64-rhs group
 
 __global__ void __launch_bounds__(256) em_bh_weights(prh_operator op, prh_schedule sch,
                                                      prh_state st, prh_em em, int64_t sS,
@@ -1433,60 +1432,45 @@ __global__ void __launch_bounds__(256) em_bh_weights_bins(prh_operator op, prh_s
   em.wbh[(int64_t)lane * K + k] = em.bhw[(int64_t)w * d + lane] * tot;
 }
 
-__global__ void __launch_bounds__(EM_THREADS) em_step_kernel(prh_operator op, prh_schedule sch,
-                                                            prh_state st, prh_em em,
-                                                            int step_local, int use_ctl) {
-  __shared__ double g0[PRH_EM_MAX * WAVE];
-  __shared__ double ga[PRH_EM_MAX * WAVE];
+// Photon ETD2 step of every column (gam -> gam2): one workgroup per column,
+// one thread per EM bin, both stages through LDS (K workgroups: the grid
+// fills the chip).  The stencil, phi factors and update in the order of the
+// nuclear stage epilogue.
+__global__ void __launch_bounds__(256) em_photon_kernel(prh_operator op, prh_schedule sch,
+                                                       prh_state st, prh_em em, int step_local,
+                                                       int use_ctl) {
+  __shared__ double g0[PRH_EM_MAX];
+  __shared__ double ga[PRH_EM_MAX];
   int step;
   bool live;
   wide_step(st, step_local, use_ctl, step, live);
   if (!live) return;
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = wave_id();
   const int K = op.K;
-  const int d = op.d;
   const int de = em.d_em;
-  // workgroup = (64-rhs group, chunk of output rows of the transfer); every
-  // chunk redoes the photon step and the pair spectrum for its group in LDS
-  const int grp = blockIdx.x / EM_RC;
-  const int chunk = blockIdx.x % EM_RC;
-  const int rows = (de + EM_RC - 1) / EM_RC;
-  const int r0 = chunk * rows;
-  const int r1 = min(de, r0 + rows);
-  const int k = grp * WAVE + lane;
-  const bool act = k < K;
-  // the photon state alternates between gam (read on even steps) and gam2
-  const double* __restrict__ gin = (step & 1) ? em.gam2 : em.gam;
-  double* __restrict__ gout = (step & 1) ? em.gam : em.gam2;
-  const int w = sch.win[step];
+  const int k = blockIdx.x;
+  const int i = threadIdx.x;
   const double hh = sch.h[step];
   const double dl = sch.dldz[step];
   const double had = sch.hadia[step];
   const bool pdiff = (op.flags & PRH_F_PARTIAL_DIFF) != 0;
-  for (int i = wv; i < de; i += EM_WAVES) g0[i * WAVE + lane] = act ? gin[(int64_t)i * K + k] : 0.0;
+  if (i < de) g0[i] = em.gam[(int64_t)i * K + k];
   __syncthreads();
-
   // F(y)_i = dl * sum_{o != 0} D[i][i+o] kappa_{i+o} y_{i+o},  kappa = H cm2sec E
-  auto stencil = [&](const double* y, int i) {
+  auto stencil = [&](const double* y) {
     double f = 0.0;
     if (pdiff) {
 #pragma unroll
       for (int o = -3; o <= 3; ++o) {
         const int j = i + o;
         if (o == 0 || j < 0 || j >= de) continue;
-        const double c = em.dcoef[i * 7 + o + 3];
-        if (c != 0.0) f += c * ((had * em.egrid[j]) * y[j * WAVE + lane]);
+        const double cf = em.dcoef[i * 7 + o + 3];
+        if (cf != 0.0) f += cf * ((had * em.egrid[j]) * y[j]);
       }
     }
     return f * dl;
   };
-  double f0[EM_NB], e_[EM_NB], p1[EM_NB], p2[EM_NB];
-#pragma unroll
-  for (int t = 0; t < EM_NB; ++t) {
-    const int i = wv + EM_WAVES * t;
-    f0[t] = e_[t] = p1[t] = p2[t] = 0.0;
-    if (i >= de) continue;
+  double f0 = 0.0, p2 = 0.0;
+  if (i < de) {
     double dv = 0.0;
     if (pdiff) {
       double kx = (had * em.egrid[i]) * em.ddiag[i];
@@ -1494,110 +1478,97 @@ __global__ void __launch_bounds__(EM_THREADS) em_step_kernel(prh_operator op, pr
     }
     const double hd = dv * hh;
     const double e = exp(hd);
-    e_[t] = e;
-    p1[t] = phi1_of(hd, e);
-    p2[t] = phi2_of(hd, e);
-    const double f = stencil(g0, i);
-    f0[t] = f;
-    double a = e * g0[i * WAVE + lane];
-    double sc = p1[t] * f;
+    const double p1 = phi1_of(hd, e);
+    p2 = phi2_of(hd, e);
+    f0 = stencil(g0);
+    double a = e * g0[i];
+    double sc = p1 * f0;
     sc = sc * hh;
-    ga[i * WAVE + lane] = a + sc;
+    ga[i] = a + sc;
   }
   __syncthreads();
-  double xn[EM_NB];
-#pragma unroll
-  for (int t = 0; t < EM_NB; ++t) {
-    const int i = wv + EM_WAVES * t;
-    xn[t] = 0.0;
-    if (i >= de) continue;
-    const double fa = stencil(ga, i);
-    double sc = fa - f0[t];
+  if (i < de) {
+    const double fa = stencil(ga);
+    double sc = fa - f0;
     sc = sc * hh;
-    sc = sc * p2[t];
-    xn[t] = ga[i * WAVE + lane] + sc;
+    sc = sc * p2;
+    em.gam2[(int64_t)i * K + k] = ga[i] + sc;
   }
-  // Bethe-Heitler pairs: e_src = R_BH w, lep = e- + e+ = 2 e_src.  The wave's
-  // rows of R_BH come in with one vector load each (lane = j) and are
-  // broadcast with readlane, instead of one scalar load per coefficient.
-  double es[EM_NB], Rr[EM_NB];
-  const double* R = em.rbh + (int64_t)w * de * d;
+}
+
+// Transfer of the step as one FP64 MFMA product per 16 x 16 tile:
+//   gam[i][k] = sum_c S[w][i][c] y[c][k],  S = [T_gamma | 2 T_electron R_BH],
+//   y = [gam2 (photons after the ETD2 step); wbh (Bethe-Heitler weights)]
+// one wave per (row tile, column tile), contraction length d_em + d.
+__global__ void __launch_bounds__(256) em_transfer_mfma(prh_operator op, prh_schedule sch,
+                                                       prh_state st, prh_em em, int step_local,
+                                                       int use_ctl) {
+  int step;
+  bool live;
+  wide_step(st, step_local, use_ctl, step, live);
+  if (!live) return;
+  const int K = op.K;
+  const int de = em.d_em;
+  const int kd = de + op.d;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int64_t wave = (int64_t)blockIdx.x * 4 + wave_id();
+  const int nct = (K + 15) / 16;
+  const int nrt = (de + 15) / 16;
+  if (wave >= (int64_t)nrt * nct) return;
+  const int rt = (int)(wave / nct), ct = (int)(wave % nct);
+  const int col = lane & 15, kq = lane >> 4;
+  const int i = rt * 16 + col;
+  const int kc = ct * 16 + col;
+  const int w = sch.win[step];
+  const double* S = em.tstack + (int64_t)w * de * kd + (int64_t)(i < de ? i : de - 1) * kd;
+  const int kcc = kc < K ? kc : K - 1;
+  double4_t acc = {0.0, 0.0, 0.0, 0.0};
+  // the loads of 10 k-steps are issued before their products
+  constexpr int UK = 10;
+  for (int c0 = 0; c0 < kd; c0 += 4 * UK) {
+    double a[UK], b[UK];
 #pragma unroll
-  for (int t = 0; t < EM_NB; ++t) {
-    const int i = wv + EM_WAVES * t;
-    es[t] = 0.0;
-    Rr[t] = (i < de && lane < d) ? R[(int64_t)i * d + lane] : 0.0;
-  }
-  // the weights of 8 bins are loaded together before they are used
-  for (int j0 = 0; j0 < d; j0 += 8) {
-    double wj[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int j = j0 + u < d ? j0 + u : d - 1;
-      wj[u] = act ? em.wbh[(int64_t)j * K + k] : 0.0;
+    for (int u = 0; u < UK; ++u) {
+      const int cc = c0 + 4 * u + kq;
+      a[u] = (i < de && cc < kd) ? S[cc] : 0.0;
+      b[u] = 0.0;
+      if (cc < de)
+        b[u] = em.gam2[(int64_t)cc * K + kcc];
+      else if (cc < kd)
+        b[u] = em.wbh[(int64_t)(cc - de) * K + kcc];
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (j0 + u >= d) break;  // uniform
-#pragma unroll
-      for (int t = 0; t < EM_NB; ++t) {
-        const int i = wv + EM_WAVES * t;
-        if (i < de) es[t] += readlane_f64(Rr[t], j0 + u) * wj[u];
-      }
-    }
+    for (int u = 0; u < UK; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
   }
-  __syncthreads();  // every wave is done reading g0 and ga
+  if (kc >= K) return;
 #pragma unroll
-  for (int t = 0; t < EM_NB; ++t) {
-    const int i = wv + EM_WAVES * t;
-    if (i >= de) continue;
-    g0[i * WAVE + lane] = xn[t];
-    ga[i * WAVE + lane] = es[t] + es[t];
+  for (int v = 0; v < 4; ++v) {
+    const int row = rt * 16 + kq + 4 * v;
+    if (row < de) em.gam[(int64_t)row * K + kc] = acc[v];
   }
-  __syncthreads();
-  // cascade transfer of this chunk's rows: gam <- T_gamma gam + T_electron lep
-  const double* Tg = em.tg + (int64_t)w * de * de;
-  const double* Te = em.te + (int64_t)w * de * de;
-  constexpr int NR = (PRH_EM_MAX / EM_RC + EM_WAVES) / EM_WAVES;
-  constexpr int NC = (PRH_EM_MAX + WAVE - 1) / WAVE;
-  // the wave's transfer rows with one vector load per 64 columns (lane = m
-  // within the slice), broadcast with readlane
-  double ag[NR], ae[NR], tg[NR][NC], te[NR][NC];
-#pragma unroll
-  for (int t = 0; t < NR; ++t) {
-    const int i = r0 + wv + EM_WAVES * t;
-    ag[t] = ae[t] = 0.0;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const int m = c * WAVE + lane;
-      const bool ok = i < r1 && m < de;
-      tg[t][c] = ok ? Tg[(int64_t)i * de + m] : 0.0;
-      te[t][c] = ok ? Te[(int64_t)i * de + m] : 0.0;
-    }
+}
+
+// [T_gamma | 2 T_electron R_BH] per window (one thread per output value,
+// the T_electron R_BH sum over the EM bins in order).
+__global__ void em_stack_kernel(const double* __restrict__ tg, const double* __restrict__ te,
+                                const double* __restrict__ rbh, int nwin, int de, int d,
+                                double* __restrict__ out) {
+  const int kd = de + d;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nwin * de * kd) return;
+  const int64_t w = t / ((int64_t)de * kd);
+  const int rem = (int)(t - w * de * kd);
+  const int i = rem / kd, c = rem % kd;
+  if (c < de) {
+    out[t] = tg[(w * de + i) * de + c];
+    return;
   }
-#pragma unroll
-  for (int c = 0; c < NC; ++c) {
-    const int mend = min(de - c * WAVE, WAVE);
-    for (int l = 0; l < mend; ++l) {
-      const int m = c * WAVE + l;
-      const double gm = g0[m * WAVE + lane];
-      const double lm = ga[m * WAVE + lane];
-#pragma unroll
-      for (int t = 0; t < NR; ++t) {
-        const int i = r0 + wv + EM_WAVES * t;
-        if (i < r1) {
-          ag[t] += readlane_f64(tg[t][c], l) * gm;
-          ae[t] += readlane_f64(te[t][c], l) * lm;
-        }
-      }
-    }
-  }
-  if (!act) return;
-#pragma unroll
-  for (int t = 0; t < NR; ++t) {
-    const int i = r0 + wv + EM_WAVES * t;
-    if (i < r1) gout[(int64_t)i * K + k] = ag[t] + ae[t];
-  }
+  const int j = c - de;
+  const double* ter = te + (w * de + i) * de;
+  const double* rb = rbh + w * de * d + j;
+  double s = 0.0;
+  for (int m = 0; m < de; ++m) s += ter[m] * rb[(int64_t)m * d];
+  out[t] = s + s;
 }
 
 void launch_em(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
@@ -1621,8 +1592,11 @@ void launch_em(const prh_operator& op, const prh_schedule& sch, const prh_state&
   else
     hipLaunchKernelGGL(em_bh_weights_bins, dim3((unsigned)op.K), dim3(256), 0, s, op, sch, st, em,
                        step_local, use_ctl);
-  hipLaunchKernelGGL(em_step_kernel, dim3((unsigned)(ng * EM_RC)), dim3(EM_THREADS), 0, s, op, sch, st, em,
+  hipLaunchKernelGGL(em_photon_kernel, dim3((unsigned)op.K), dim3(192), 0, s, op, sch, st, em,
                      step_local, use_ctl);
+  const int64_t tiles = (int64_t)((em.d_em + 15) / 16) * ((op.K + 15) / 16);
+  hipLaunchKernelGGL(em_transfer_mfma, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, op, sch,
+                     st, em, step_local, use_ctl);
 }
 
 int32_t check_uneven(const prh_operator* op, const prh_uneven* un, const prh_state* st,
@@ -1649,7 +1623,7 @@ int32_t validate_em(const prh_operator* op, const prh_state* st, const char* who
   if (em->d_em < 1 || em->d_em > PRH_EM_MAX)
     return set_error(std::string(who) + ": d_em must be in [1, 144]");
   if (!em->egrid || !em->dcoef || !em->ddiag || !em->cbh || !em->bhw || !em->rbh || !em->tg ||
-      !em->te || !em->gam || !em->gam2 || !em->wbh)
+      !em->te || !em->gam || !em->gam2 || !em->wbh || !em->tstack)
     return set_error(std::string(who) + ": EM sector with a null buffer");
   if (em->nwin < 1) return set_error(std::string(who) + ": EM sector without windows");
   (void)op;
@@ -1921,7 +1895,7 @@ void launch_uneven(const prh_operator& op, const prh_uneven& un, const prh_state
 // selects T_gamma, T_electron, R_BH and the Bethe-Heitler weights; its own
 // h, dl/dz and H cm2sec drive the photon ETD2 step.  The photon state
 // [d_em][K] is updated in place (the whole column lives in one workgroup).
-// Same sums in the same order as em_bh_weights_bins + em_step_kernel.
+// The Bethe-Heitler weights sum the species as em_bh_weights_bins does.
 __global__ void __launch_bounds__(256) em_uneven_kernel(prh_operator op, prh_uneven un,
                                                        prh_state st, prh_em em, int step) {
   __shared__ double part[4][WAVE];
@@ -2331,6 +2305,8 @@ constexpr int CTHREADS = CWAVES * WAVE;
 constexpr int CTILES = CTL * CTL;     // 81 output tiles
 constexpr int CTPW = (CTILES + CWAVES - 1) / CWAVES;  // tiles per wave (6)
 constexpr int64_t CMAT = (int64_t)CNP * CNP;
+constexpr int CKU = 4;                // k-steps whose loads are issued together (36 = 9 x 4)
+constexpr int CKU2 = 2;               // the same for the three-product pass (register budget)
 enum CascadeMat { CM_P = 0, CM_SG, CM_DE, CM_MIC, CM_G, CM_E, CM_E1, CM_ESC, CM_COUNT };
 
 // rows of the contraction output per window (prh_cascade layout)
@@ -2477,12 +2453,21 @@ __global__ void __launch_bounds__(CTHREADS) cascade_transfer_kernel(
       if (t >= CTILES) break;
       const int r0 = (t / CTL) * 16, c0 = (t % CTL) * 16;
       double4_t acc = {0.0, 0.0, 0.0, 0.0};
-      for (int k0 = 0; k0 < CNP; k0 += 4) {
-        const int k = k0 + kq;
-        const double a = P[(int64_t)(r0 + col) * CNP + k];
-        const double g = G[(int64_t)k * CNP + c0 + col];
-        const double b = (g - spesc[k] * g) * sdE[k];
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      // the operands of CKU k-steps are loaded before their products
+      for (int k0 = 0; k0 < CNP; k0 += 4 * CKU) {
+        double a[CKU], g[CKU];
+#pragma unroll
+        for (int u = 0; u < CKU; ++u) {
+          const int k = k0 + 4 * u + kq;
+          a[u] = P[(int64_t)(r0 + col) * CNP + k];
+          g[u] = G[(int64_t)k * CNP + c0 + col];
+        }
+#pragma unroll
+        for (int u = 0; u < CKU; ++u) {
+          const int k = k0 + 4 * u + kq;
+          const double b = (g[u] - spesc[k] * g[u]) * sdE[k];
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b, acc, 0, 0, 0);
+        }
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -2500,15 +2485,26 @@ __global__ void __launch_bounds__(CTHREADS) cascade_transfer_kernel(
       if (t >= CTILES) break;
       const int r0 = (t / CTL) * 16, c0 = (t % CTL) * 16;
       double4_t gc = {0.0, 0.0, 0.0, 0.0}, gh = gc, en = gc;
-      for (int k0 = 0; k0 < CNP; k0 += 4) {
-        const int k = k0 + kq;
-        const int64_t ra = (int64_t)(r0 + col) * CNP + k;
-        const double ek = E1[(int64_t)k * CNP + c0 + col];
-        const double yc = scold[k] != 0.0 ? ek * sdE[k] : 0.0;
-        const double yh = scold[k] != 0.0 ? 0.0 : ek * sdE[k];
-        gc = __builtin_amdgcn_mfma_f64_16x16x4f64(Mic[ra], yc, gc, 0, 0, 0);
-        gh = __builtin_amdgcn_mfma_f64_16x16x4f64(Sg[ra], yh, gh, 0, 0, 0);
-        en = __builtin_amdgcn_mfma_f64_16x16x4f64(De[ra], yh, en, 0, 0, 0);
+      for (int k0 = 0; k0 < CNP; k0 += 4 * CKU2) {
+        double am[CKU2], as[CKU2], ad[CKU2], ek[CKU2];
+#pragma unroll
+        for (int u = 0; u < CKU2; ++u) {
+          const int k = k0 + 4 * u + kq;
+          const int64_t ra = (int64_t)(r0 + col) * CNP + k;
+          am[u] = Mic[ra];
+          as[u] = Sg[ra];
+          ad[u] = De[ra];
+          ek[u] = E1[(int64_t)k * CNP + c0 + col];
+        }
+#pragma unroll
+        for (int u = 0; u < CKU2; ++u) {
+          const int k = k0 + 4 * u + kq;
+          const double yc = scold[k] != 0.0 ? ek[u] * sdE[k] : 0.0;
+          const double yh = scold[k] != 0.0 ? 0.0 : ek[u] * sdE[k];
+          gc = __builtin_amdgcn_mfma_f64_16x16x4f64(am[u], yc, gc, 0, 0, 0);
+          gh = __builtin_amdgcn_mfma_f64_16x16x4f64(as[u], yh, gh, 0, 0, 0);
+          en = __builtin_amdgcn_mfma_f64_16x16x4f64(ad[u], yh, en, 0, 0, 0);
+        }
       }
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
@@ -2899,6 +2895,18 @@ int32_t prh_graph_destroy(prh_graph* g) {
   return 0;
 }
 
+
+int32_t prh_em_transfer_stack(const double* tg, const double* te, const double* rbh,
+                              int32_t nwin, int32_t d_em, int32_t d, double* out, void* stream) {
+  if (!tg || !te || !rbh || !out) return set_error("prh_em_transfer_stack: null argument");
+  if (d_em < 1 || d_em > PRH_EM_MAX || d < 1 || d > WAVE || nwin < 0)
+    return set_error("prh_em_transfer_stack: bad sizes");
+  const int64_t total = (int64_t)nwin * d_em * (d_em + d);
+  if (total == 0) return 0;
+  hipLaunchKernelGGL(em_stack_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, tg, te, rbh, (int)nwin, (int)d_em, (int)d, out);
+  return check_hip(hipGetLastError(), "prh_em_transfer_stack launch");
+}
 
 int32_t prh_toeplitz_rows(const prh_toeplitz* tz, const int64_t* sel, int64_t nsel, double* out,
                           void* stream) {

@@ -431,11 +431,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
     def em_host_state(self, plan=None):
         """EM blocks (e+, e-, photon) of the last solve as a host array [3 d_em, K]."""
         plan = plan if plan is not None else self._plan
-        # the photon state alternates between two buffers: after step s it is in
-        # gam2 when s is even
-        last = plan["nsteps"] - 1
-        key = "gam2" if last % 2 == 0 else "gam"
-        gam = plan["em_bufs"][key].cpu().numpy()
+        gam = plan["em_bufs"]["gam"].cpu().numpy()
         z = np.zeros_like(gam)
         return np.concatenate([z, z, gam], axis=0)
 
@@ -529,11 +525,37 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         return rest, comp, scales
 
     def _refresh_z_caches(self, zs):
-        """Host inputs of every refresh window: photon vectors and pair rates.
+        """Inputs of every refresh window: photon vectors and pair rates.
 
         Returns (phot, rpair, phot_e, rpair_e); the last two are None unless a
-        per-column field scale is set.
+        per-column field scale is set.  For a CMB [+ synthetic EBL] field they
+        are evaluated on the device (``DeviceFieldTables``, tensors in HBM);
+        other fields are evaluated by their host models.
         """
+        from .field_tables import DeviceFieldTables, device_field
+
+        parts = device_field(self.prince_run.photon_field)
+        split = self._split_fields()
+        if parts is not None and (split is None or (parts[1] is not None
+                                                    and split[1] is parts[1])):
+            ft = getattr(self.prince_run, "_prh_field_tables", None)
+            if ft is None or ft.device != self.device:
+                ft = DeviceFieldTables(self.prince_run, self.device)
+                self.prince_run._prh_field_tables = ft
+            if split is None:
+                phot, rpair = ft.tables(zs)
+                self._rpair_raw = rpair.clone()
+                if not self.enable_pairprod_losses:
+                    rpair.zero_()
+                return phot, rpair, None, None
+            self._rpair_raw = None
+            phot, rpair = ft.tables(zs, ebl=False)
+            phot_e, rpair_e = ft.tables(zs, cmb=False)
+            if not self.enable_pairprod_losses:
+                rpair.zero_()
+                rpair_e.zero_()
+            return phot, rpair, phot_e, rpair_e
+        self._rpair_raw = None
         ir = self.had_int_rates
         pair = self.pair_loss_rates_grid
         d = self.prince_run.cr_grid.d
@@ -613,6 +635,8 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         phot, rpair, phot_e, rpair_e = self._refresh_z_caches(zs)
         two = phot_e is not None
         phot0 = phot[0] + phot_e[0] if two else phot[0]
+        if not isinstance(phot0, np.ndarray):
+            phot0 = phot0.cpu().numpy()
         lay = self._layout_for(cache, phot0, with_transpose=with_adjoint)
         wl = lay["layout"]
         nwin = len(zs)
@@ -664,6 +688,8 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         pscale = pair_species_scale(self.pair_loss_rates_grid, self.spec_man)
 
         def dv(a, dtype=None):
+            if isinstance(a, torch.Tensor):
+                return a.contiguous()
             a = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype=dtype))
             return torch.from_numpy(a).to(dev)
 
@@ -808,6 +834,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
 
         torch = _torch()
         run = self.prince_run
+        dev = self.device
         if self.enable_decay:
             raise NotImplementedError("explicit decay with the EM cascade")
         if self._split_fields() is not None:
@@ -826,18 +853,22 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         E_p = run.cr_grid.grid
         dE = np.gradient(E_p)
         pair = self.pair_loss_rates_grid
-        bhw = np.stack([
-            (PRINCE_UNITS.c * self._em_dz / ((1.0 + z) * H(z))) * (dE / E_p)
-            * pair_rate_per_energy(pair, z)
-            for z in zs
-        ])
+        # Bethe-Heitler weights dl_step (dE/E) rate(z) per window, with the pair
+        # rate whether or not the nuclear pair loss is on (the reference's hook)
+        dl = np.array([PRINCE_UNITS.c * self._em_dz / ((1.0 + z) * H(z)) for z in zs])
+        raw = getattr(self, "_rpair_raw", None)
+        if raw is not None:
+            bhw = (torch.from_numpy(dl).to(dev)[:, None]
+                   * torch.from_numpy(dE / E_p).to(dev)[None, :]) * raw
+        else:
+            bhw = np.stack([dl[w] * (dE / E_p) * pair_rate_per_energy(pair, z)
+                            for w, z in enumerate(zs)])
         pscale = pair_species_scale(pair, self.spec_man)
         cbh = np.zeros(self.spec_man.nspec)
         for sref in self.spec_man.species_refs:
             if sref.is_nucleus and abs(sref.charge) >= 1:
                 cbh[sref.princeidx] = float(sref.A) * pscale[sref.princeidx]
         dcoef, ddiag = fd_tables(single_block(eg))
-        dev = self.device
         gam0 = np.zeros((eg.d, K))
         init = getattr(self, "_em_init", None)
         if init is not None and init.size:
@@ -850,11 +881,17 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             "gam2": torch.zeros((eg.d, K), dtype=torch.float64, device=dev),
             "wbh": torch.zeros((run.cr_grid.d, K), dtype=torch.float64, device=dev),
         }
+        d = run.cr_grid.d
+        b["tstack"] = torch.empty((len(zs), eg.d, eg.d + d), dtype=torch.float64, device=dev)
+        _lib.check(self._lib.prh_em_transfer_stack(
+            Tg.data_ptr(), Te.data_ptr(), Rbh.data_ptr(), len(zs), eg.d, d,
+            b["tstack"].data_ptr(), torch.cuda.current_stream(dev).cuda_stream), "EM transfer stack")
         em = _lib.PrhEm(
             d_em=eg.d, nwin=len(zs), egrid=b["egrid"].data_ptr(), dcoef=b["dcoef"].data_ptr(),
             ddiag=b["ddiag"].data_ptr(), cbh=b["cbh"].data_ptr(), bhw=b["bhw"].data_ptr(),
             rbh=b["rbh"].data_ptr(), tg=b["tg"].data_ptr(), te=b["te"].data_ptr(),
             gam=b["gam"].data_ptr(), wbh=b["wbh"].data_ptr(), gam2=b["gam2"].data_ptr(),
+            tstack=b["tstack"].data_ptr(),
         )
         self.em_iterations = iters.cpu().numpy().tolist()
         return em, b

@@ -109,37 +109,9 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
 
     def _device_tables(self, zw, sw, dev):
         """Photon vectors [nwin][dph] and pair rates [nwin][d] of all column windows."""
-        torch = _torch()
-        run = self.prince_run
-        _, ebl = _field_parts(run.photon_field)
-        ir = self.had_int_rates
-        pair = self.pair_loss_rates_grid
-        stream = torch.cuda.current_stream(dev).cuda_stream
-        zt = torch.from_numpy(np.ascontiguousarray(zw)).to(dev)
-        st = torch.from_numpy(np.ascontiguousarray(sw * ebl.scale)).to(dev)
-        nw = zw.size
+        from .field_tables import DeviceFieldTables
 
-        def table(P, W, E0, ni, nm):
-            Pt = torch.from_numpy(np.ascontiguousarray(P, dtype=np.float64)).to(dev)
-            Wt = torch.from_numpy(np.ascontiguousarray(W, dtype=np.float64)).to(dev)
-            Et = torch.from_numpy(np.ascontiguousarray(E0, dtype=np.float64)).to(dev)
-            out = torch.empty((nw, ni), dtype=torch.float64, device=dev)
-            _lib.check(self._lib.prh_field_tables(
-                zt.data_ptr(), st.data_ptr(), nw, Pt.data_ptr(), Wt.data_ptr(), Et.data_ptr(),
-                ni, nm, PLANCK_PREF, mconfig.E_CMB, SyntheticEBL.Z_POWER, out.data_ptr(), stream),
-                "field tables")
-            return out
-
-        Eph = ir.e_photon.grid
-        phot = table(Eph, np.ones_like(Eph), ebl.density_z0(Eph), Eph.size, 1)
-        pg = pair.photon_grid  # [d][nxi]
-        tw = np.empty_like(pair.xi)
-        tw[1:-1] = 0.5 * (pair.xi[2:] - pair.xi[:-2])
-        tw[0] = 0.5 * (pair.xi[1] - pair.xi[0])
-        tw[-1] = 0.5 * (pair.xi[-1] - pair.xi[-2])
-        W = np.broadcast_to(tw * pair.phi_xi2, pg.shape)
-        rpair = table(pg, W, ebl.density_z0(pg), pg.shape[0], pg.shape[1])
-        return phot, rpair
+        return DeviceFieldTables(self.prince_run, dev).tables(zw, scales=sw)
 
     def prepare(self, dz=1e-3, with_adjoint=False):
         if with_adjoint:
