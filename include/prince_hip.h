@@ -173,8 +173,16 @@ typedef struct prh_graph prh_graph;
 /* Uneven batch: every right-hand side k has its own z grid (its own z_max),
  * refresh windows and coupling values.  Per-step tables are [nsteps][K];
  * win < 0 marks a finished column (its state is left as it is).  mrhs holds
- * each column's current window values in the window layout of prh_operator
- * ([K][mvals_stride], diagonal at diag_off), refreshed by prh_fold_rows. */
+ * each column's window values in the window layout of prh_operator
+ * ([nslot][K][mvals_stride], diagonal at diag_off), written by prh_fold_rows:
+ * with nslot = 1 a column's only slot is re-folded at the step its window
+ * starts; with nslot = 2 window w (a row of rpair) lives in slot w & 1, so
+ * the next window of a column can be folded while its current one is in use
+ * (windows of one column must then have consecutive ids).  live, a HOST
+ * array [nsteps] or NULL (all K), is the number of columns with step t: the
+ * columns with win >= 0 at step t must be 0 .. live[t] - 1 (columns ordered
+ * by decreasing step count), and only those are launched -- finished columns
+ * retire from the grid. */
 typedef struct prh_uneven {
   int32_t nsteps;        /* steps of the longest column */
   int32_t nwin;          /* windows over all columns (rows of rpair) */
@@ -185,12 +193,17 @@ typedef struct prh_uneven {
   const int32_t* win;    /* [nsteps][K] window id (row of rpair) or -1 */
   const double* src_coef;/* [nsteps][J][K] norm * evolution(z_k) per source */
   const double* rpair;   /* [nwin][d] pair-production rate of each column window */
-  const double* mrhs;    /* [K][mvals_stride] current window values of each column */
+  const double* mrhs;    /* [nslot][K][mvals_stride] window values of each column */
+  int32_t nslot;         /* 1 or 2 value slots per column */
+  int32_t pad_;
+  const int32_t* live;   /* HOST [nsteps]: live columns of each step, or NULL */
 } prh_uneven;
 
-/* Per-step re-fold lists of the uneven batch for graph capture: at step t the
- * columns out_rows[ref_off[t] .. ref_off[t+1]) are re-folded from the photon
- * rows in_rows[...] (device arrays); ref_off is a HOST array of nsteps + 1. */
+/* Per-step re-fold lists of the uneven batch for graph capture: before step t
+ * the value rows out_rows[ref_off[t] .. ref_off[t+1]) of mrhs (slot * K +
+ * column) are folded from the photon rows in_rows[...] (device arrays);
+ * ref_off is a HOST array of nsteps + 1.  With two slots the windows starting
+ * in a group of steps can all be folded before its first step. */
 typedef struct prh_refold {
   const double* kern;      /* [nrows][dph] window-layout fold kernel */
   int64_t nrows;
@@ -332,7 +345,7 @@ int32_t prh_fold_rows(const double* kern, int64_t nrows, int32_t dph, const doub
 
 /* Both ETD2 stages of step `step` of the uneven batch (narrow layout, kt 1/2/4).
  * Each column follows its own reference solve (propagation.py:813-890 with
- * etd2.py:412-439); finished columns are skipped.  With st->em set, the EM
+ * etd2.py:412-439); only the un->live[step] leading columns are launched.  With st->em set, the EM
  * step of every live column follows (its own window's T_gamma, T_electron,
  * R_BH and Bethe-Heitler weights: tables indexed by the column windows of
  * un->win, em->nwin == un->nwin; the photon state em->gam [d_em][K] is

@@ -159,6 +159,9 @@ class PrhUneven(ctypes.Structure):
         ("src_coef", _p),
         ("rpair", _p),
         ("mrhs", _p),
+        ("nslot", _i32),
+        ("pad_", _i32),
+        ("live", _p),
     ]
 
 

@@ -1613,6 +1613,9 @@ int32_t check_uneven(const prh_operator* op, const prh_uneven* un, const prh_sta
   if (op->ndense > 0) return set_error(w + ": the uneven kernel has no dense-row pass");
   if (!un->h || !un->dldz || !un->hadia || !un->finj || !un->win || !un->rpair || !un->mrhs)
     return set_error(w + ": null schedule buffer");
+  if (un->nslot != 1 && un->nslot != 2) return set_error(w + ": nslot must be 1 or 2");
+  if ((int64_t)op->mvals_stride * 8 > INT32_MAX || (int64_t)op->K * op->d * 8 > INT32_MAX)
+    return set_error(w + ": value rows and state panels must stay below 2 GiB");
   if ((op->flags & PRH_F_INJECTION) && !un->src_coef) return set_error(w + ": injection needs src_coef");
   return 0;
 }
@@ -1690,104 +1693,232 @@ __global__ void __launch_bounds__(THREADS) fold_rows_mfma(const double* __restri
     const int j = 4 * t + kq;
     bfrag[t] = (t < nk && j < dph && r < nrows) ? kern[r * dph + j] : 0.0;
   }
-  for (int c0 = 0; c0 < ncol; c0 += 16) {
-    double4_t acc = {0.0, 0.0, 0.0, 0.0};
-    const int ca = c0 + col;
-    const int64_t prow = ca < ncol ? (int64_t)irow[ca] * dph : 0;
+  // two 16-column chunks per pass: two independent MFMA chains, and the photon
+  // rows of both gathered before either chain starts
+  for (int c0 = 0; c0 < ncol; c0 += 32) {
+    double4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
+    const int ca0 = c0 + col, ca1 = c0 + 16 + col;
+    const int64_t prow0 = ca0 < ncol ? (int64_t)irow[ca0] * dph : 0;
+    const int64_t prow1 = ca1 < ncol ? (int64_t)irow[ca1] * dph : 0;
 #pragma unroll
     for (int t = 0; t < FOLD_MAX_KSTEPS; ++t) {
       if (t < nk) {
         const int j = 4 * t + kq;
-        const double a = (ca < ncol && j < dph) ? phot[prow + j] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bfrag[t], acc, 0, 0, 0);
+        const double a0 = (ca0 < ncol && j < dph) ? phot[prow0 + j] : 0.0;
+        const double a1 = (ca1 < ncol && j < dph) ? phot[prow1 + j] : 0.0;
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bfrag[t], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bfrag[t], acc1, 0, 0, 0);
       }
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int c = c0 + kq + 4 * v;
-      if (c < ncol && r < nrows) out[(int64_t)orow[c] * out_stride + r] = acc[v];
+      if (c < ncol && r < nrows) out[(int64_t)orow[c] * out_stride + r] = acc0[v];
+      if (c + 16 < ncol && r < nrows) out[(int64_t)orow[c + 16] * out_stride + r] = acc1[v];
     }
   }
 }
 
+// Coupling product of one row part for KT columns of the uneven batch, each
+// with its own window values (vbase[kk]: the column's value row in mrhs):
+//   acc[kk] = sum_b m_{b,k} (shifted) y_b[k]
+// in block order, as couple_part.  Runs of diagonal blocks and of band blocks
+// issue the loads of UB blocks before any product, through buffer resources
+// (one per column's value row, one per mother panel).
+template <int KT>
+struct UnevenBatch {
+  static constexpr int DIAG = KT == 1 ? 8 : (KT == 2 ? 6 : 3);
+};
+
+template <int KT>
+__device__ __forceinline__ void couple_cols(const prh_operator& op, const double* __restrict__ y,
+                                            const double* const (&vbase)[KT], const int (&kc)[KT],
+                                            int b_begin, int b_end, int kind, int lane,
+                                            bool active, double (&acc)[KT]) {
+  const int d = op.d;
+  const int64_t Kd = (int64_t)op.K * d;
+  const int lb = (active ? lane : d - 1) * 8;
+  const int vbytes = (int)(op.mvals_stride * 8);
+  const int pbytes = (int)(Kd * 8);
+  __amdgpu_buffer_rsrc_t rv[KT];
+  int coff[KT];
+#pragma unroll
+  for (int kk = 0; kk < KT; ++kk) {
+    rv[kk] = tile_rsrc(vbase[kk], vbytes);
+    coff[kk] = kc[kk] * d * 8;
+    acc[kk] = 0.0;
+  }
+  const int dbase = b_begin;
+  int dmo = 0, dol = 0, dns = 0, dvo = 0;
+  if (b_begin + lane < b_end) {
+    const int* blk = op.blocks + 4 * (b_begin + lane);
+    dmo = blk[0];
+    dol = blk[1];
+    dns = blk[2];
+    dvo = blk[3];
+  }
+  if (kind == PART_DIAG) {
+    constexpr int UB = UnevenBatch<KT>::DIAG;
+    for (int b = b_begin; b < b_end; b += UB) {
+      double mv[UB][KT], xv[UB][KT];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) {
+        const int bb = b + u < b_end ? b + u : b_end - 1;
+        int mo, voffi;
+        if (bb - dbase < WAVE) {
+          mo = __builtin_amdgcn_readlane(dmo, bb - dbase);
+          voffi = __builtin_amdgcn_readlane(dvo, bb - dbase);
+        } else {
+          mo = op.blocks[4 * bb];
+          voffi = op.blocks[4 * bb + 3];
+        }
+        const auto ry = tile_rsrc(y + (int64_t)mo * Kd, pbytes);
+#pragma unroll
+        for (int kk = 0; kk < KT; ++kk) {
+          mv[u][kk] = tload(rv[kk], lb, voffi * 8);
+          xv[u][kk] = tload(ry, lb, coff[kk]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UB; ++u) {
+        if (b + u < b_end) {
+#pragma unroll
+          for (int kk = 0; kk < KT; ++kk) acc[kk] += mv[u][kk] * xv[u][kk];
+        }
+      }
+    }
+  } else if (kind == PART_BAND) {
+    // one block at a time, its diagonals in two halves of NSMAX / 2 (loads of
+    // both columns issued before the products of each half)
+    constexpr int NH = NSMAX / 2;
+    for (int b = b_begin; b < b_end; ++b) {
+      int mo, ol, nsl, voffi;
+      if (b - dbase < WAVE) {
+        mo = __builtin_amdgcn_readlane(dmo, b - dbase);
+        ol = __builtin_amdgcn_readlane(dol, b - dbase);
+        nsl = __builtin_amdgcn_readlane(dns, b - dbase);
+        voffi = __builtin_amdgcn_readlane(dvo, b - dbase);
+      } else {
+        mo = op.blocks[4 * b];
+        ol = op.blocks[4 * b + 1];
+        nsl = op.blocks[4 * b + 2];
+        voffi = op.blocks[4 * b + 3];
+      }
+      const auto ry = tile_rsrc(y + (int64_t)mo * Kd, pbytes);
+      double xv[KT];
+#pragma unroll
+      for (int kk = 0; kk < KT; ++kk) {
+        const double v = tload(ry, lb, coff[kk]);
+        xv[kk] = active ? v : 0.0;
+      }
+      if (ol >= 0 && ol != 0) {
+#pragma unroll
+        for (int kk = 0; kk < KT; ++kk) xv[kk] = shfl_or_zero(xv[kk], lane + ol, d);
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (h * NH >= nsl) break;
+        double mv[KT][NH];
+#pragma unroll
+        for (int s = 0; s < NH; ++s) {
+          const int s2 = h * NH + s;
+          const int sc = s2 < nsl ? s2 : nsl - 1;
+#pragma unroll
+          for (int kk = 0; kk < KT; ++kk) mv[kk][s] = tload(rv[kk], lb, (voffi + sc * d) * 8);
+        }
+#pragma unroll
+        for (int s = 0; s < NH; ++s) {
+          const int s2 = h * NH + s;
+          if (s2 < nsl) {
+            if (ol >= 0) {
+              // diagonal s2 reads lane + ol + s2: one shuffle to the first
+              // diagonal (above), then one-lane DPP shifts
+#pragma unroll
+              for (int kk = 0; kk < KT; ++kk) acc[kk] += mv[kk][s] * xv[kk];
+              if (s2 + 1 < nsl) {
+#pragma unroll
+                for (int kk = 0; kk < KT; ++kk) xv[kk] = lane_next(xv[kk]);
+              }
+            } else {
+              const int src = lane + ol + s2;
+#pragma unroll
+              for (int kk = 0; kk < KT; ++kk) acc[kk] += mv[kk][s] * shfl_or_zero(xv[kk], src, d);
+            }
+          }
+        }
+      }
+    }
+  } else {
+    for (int b = b_begin; b < b_end; ++b) {
+      const int* blk = op.blocks + 4 * b;
+      const int mo = blk[0];
+      const int olo = blk[1];
+      const int nslot = blk[2];
+      const int voffi = blk[3];
+      const auto ry = tile_rsrc(y + (int64_t)mo * Kd, pbytes);
+      double xv[KT];
+#pragma unroll
+      for (int kk = 0; kk < KT; ++kk) {
+        const double v = tload(ry, lb, coff[kk]);
+        xv[kk] = active ? v : 0.0;
+      }
+      for (int s2 = 0; s2 < nslot; ++s2) {
+        const int src = lane + olo + s2;
+#pragma unroll
+        for (int kk = 0; kk < KT; ++kk)
+          acc[kk] += tload(rv[kk], lb, (voffi + s2 * d) * 8) * shfl_or_zero(xv[kk], src, d);
+      }
+    }
+  }
+}
+
+// One stage of the uneven batch for the live columns [0, ncols) of step
+// `step`.  Workgroup L runs on XCD L % 8; XCD x takes the column groups x,
+// x + 8, ... and walks all row groups of one column group before the next, so
+// the mother panels of the few column groups in flight stay in its L2 (the
+// coupling values are read once per stage whatever the order).
 template <bool STAGE2, int KT>
-__global__ void __launch_bounds__(STHREADS) etd2_uneven(prh_operator op, prh_uneven un,
-                                                        prh_state st, int step) {
+__global__ void __launch_bounds__(STHREADS, KT <= 2 ? 8 : 4) etd2_uneven(prh_operator op, prh_uneven un,
+                                                        prh_state st, int step, int ncols) {
   __shared__ double red[SWG][KT][WAVE];
   const int lane = threadIdx.x & (WAVE - 1);
   const int wv = wave_id();
   const int K = op.K;
   const int d = op.d;
-  int gx, gy;
-  wg_coords(op.ntasks, K, KT, gx, gy);
+  const int nx = op.ntasks / SWG;
+  const int ny = (ncols + KT - 1) / KT;
+  const int L = blockIdx.x;
+  const int j = L >> 3;
+  const int gy = (L & 7) + 8 * (j / nx);
+  const int gx = j % nx;
+  if (gy >= ny) return;  // padding of the column groups to a multiple of 8
   const int k0 = gy * KT;
-  // per-column window of this step (-1: the column has finished); a workgroup
-  // whose columns have all finished leaves before any barrier
-  int wk[KT];
+  const int kend = ncols;
+  // per-column window of this step (-1: finished); a column outside the
+  // launched range reads the group's first column (in bounds, result unused)
+  int wk[KT], kc[KT];
+  const double* vb[KT];
   bool any = false;
 #pragma unroll
   for (int kk = 0; kk < KT; ++kk) {
     const int k = k0 + kk;
-    wk[kk] = (k < K) ? un.win[(int64_t)step * K + k] : -1;
+    wk[kk] = (k < kend) ? un.win[(int64_t)step * K + k] : -1;
+    kc[kk] = wk[kk] >= 0 ? k : k0;
+    const int slot = (un.nslot == 2 && wk[kk] >= 0) ? (wk[kk] & 1) : 0;
+    vb[kk] = un.mrhs + ((int64_t)slot * K + kc[kk]) * op.mvals_stride;
     any = any || wk[kk] >= 0;
   }
-  if (!any) return;
+  if (!any) return;  // uniform over the workgroup, before any barrier
   const int* tk = op.tasks + 6 * ((int64_t)gx * SWG + wv);
   const int da = tk[0];
   const bool active = lane < d;
   const double* __restrict__ xin = STAGE2 ? st.a : st.x;
-  const int64_t stride = op.mvals_stride;
   double acc[KT];
-#pragma unroll
-  for (int kk = 0; kk < KT; ++kk) acc[kk] = 0.0;
   if (da >= 0) {
-    for (int b = tk[1]; b < tk[2]; ++b) {
-      const int* blk = op.blocks + 4 * b;
-      const int mo = blk[0];
-      const int olo = blk[1];
-      const int ns = blk[2];
-      const int64_t voff = blk[3];
-      double y[KT];
+    couple_cols<KT>(op, xin, vb, kc, tk[1], tk[2], tk[5] & PRH_TASK_KIND_MASK, lane, active, acc);
+  } else {
 #pragma unroll
-      for (int kk = 0; kk < KT; ++kk) {
-        const int k = k0 + kk;
-        y[kk] = (active && wk[kk] >= 0) ? xin[((int64_t)mo * K + k) * d + lane] : 0.0;
-      }
-      if (olo >= 0) {
-        // diagonal s2 reads lane + olo + s2: one shuffle to the first
-        // diagonal (none when olo = 0), then one-lane DPP shifts
-        if (olo != 0) {
-#pragma unroll
-          for (int kk = 0; kk < KT; ++kk) y[kk] = shfl_or_zero(y[kk], lane + olo, d);
-        }
-        for (int s2 = 0; s2 < ns; ++s2) {
-#pragma unroll
-          for (int kk = 0; kk < KT; ++kk) {
-            const int k = k0 + kk;
-            const double m = (active && wk[kk] >= 0)
-                                 ? un.mrhs[(int64_t)k * stride + voff + (int64_t)s2 * d + lane]
-                                 : 0.0;
-            acc[kk] += m * y[kk];
-          }
-          if (s2 + 1 < ns) {
-#pragma unroll
-            for (int kk = 0; kk < KT; ++kk) y[kk] = lane_next(y[kk]);
-          }
-        }
-      } else {
-        for (int s2 = 0; s2 < ns; ++s2) {
-          const int src = lane + olo + s2;
-#pragma unroll
-          for (int kk = 0; kk < KT; ++kk) {
-            const int k = k0 + kk;
-            const double m = (active && wk[kk] >= 0)
-                                 ? un.mrhs[(int64_t)k * stride + voff + (int64_t)s2 * d + lane]
-                                 : 0.0;
-            acc[kk] += m * shfl_or_zero(y[kk], src, d);
-          }
-        }
-      }
-    }
+    for (int kk = 0; kk < KT; ++kk) acc[kk] = 0.0;
   }
   reduce_parts<KT>(red, wv, lane, tk[3], tk[4], acc);
   if (da < 0 || wv != tk[3]) return;
@@ -1811,7 +1942,7 @@ __global__ void __launch_bounds__(STHREADS) etd2_uneven(prh_operator op, prh_une
     const double hh = un.h[sk];
     double kap = un.hadia[sk] * E;
     kap = kap + psc * (active ? un.rpair[(int64_t)wk[kk] * d + lane] : 0.0);
-    const double mdc = active ? un.mrhs[(int64_t)k * stride + op.diag_off + (int64_t)da * d + lane] : 0.0;
+    const double mdc = active ? vb[kk][op.diag_off + (int64_t)da * d + lane] : 0.0;
     double dval = mdc * dl;
     if (pdiff) {
       double kxd = kap * ddg;
@@ -1846,10 +1977,10 @@ __global__ void __launch_bounds__(STHREADS) etd2_uneven(prh_operator op, prh_une
     out = out * dl;
     if (inj && active) {
       double bsum = 0.0;
-      for (int j = 0; j < J; ++j) {
-        const double c = un.src_coef[((int64_t)step * J + j) * K + k];
-        const double g = op.src_grid[(((int64_t)slot * J + j) * K + k) * d + lane];
-        bsum = (j == 0) ? c * g : bsum + c * g;
+      for (int j2 = 0; j2 < J; ++j2) {
+        const double c = un.src_coef[((int64_t)step * J + j2) * K + k];
+        const double g = op.src_grid[(((int64_t)slot * J + j2) * K + k) * d + lane];
+        bsum = (j2 == 0) ? c * g : bsum + c * g;
       }
       out = out + un.finj[sk] * bsum;
     }
@@ -1874,18 +2005,19 @@ __global__ void __launch_bounds__(STHREADS) etd2_uneven(prh_operator op, prh_une
 
 template <bool S2>
 void launch_uneven(const prh_operator& op, const prh_uneven& un, const prh_state& st, int step,
-                   hipStream_t s) {
-  const int64_t nwg = (int64_t)(op.ntasks / SWG) * ((op.K + op.kt - 1) / op.kt);
+                   int ncols, hipStream_t s) {
+  const int64_t ny = (ncols + op.kt - 1) / op.kt;
+  const int64_t nwg = (int64_t)(op.ntasks / SWG) * ((ny + 7) / 8 * 8);
   const dim3 grid((unsigned)nwg);
   switch (op.kt) {
     case 1:
-      hipLaunchKernelGGL((etd2_uneven<S2, 1>), grid, dim3(STHREADS), 0, s, op, un, st, step);
+      hipLaunchKernelGGL((etd2_uneven<S2, 1>), grid, dim3(STHREADS), 0, s, op, un, st, step, ncols);
       break;
     case 2:
-      hipLaunchKernelGGL((etd2_uneven<S2, 2>), grid, dim3(STHREADS), 0, s, op, un, st, step);
+      hipLaunchKernelGGL((etd2_uneven<S2, 2>), grid, dim3(STHREADS), 0, s, op, un, st, step, ncols);
       break;
     default:
-      hipLaunchKernelGGL((etd2_uneven<S2, 4>), grid, dim3(STHREADS), 0, s, op, un, st, step);
+      hipLaunchKernelGGL((etd2_uneven<S2, 4>), grid, dim3(STHREADS), 0, s, op, un, st, step, ncols);
       break;
   }
 }
@@ -3050,11 +3182,14 @@ int32_t prh_uneven_step(const prh_operator* op, const prh_uneven* un, const prh_
     if (em->nwin != un->nwin)
       return set_error("prh_uneven_step: the EM tables need one entry per column window");
   }
+  const int32_t live = un->live ? un->live[step] : op->K;
+  if (live < 0 || live > op->K) return set_error("prh_uneven_step: live column count outside [0, K]");
+  if (live == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  launch_uneven<false>(*op, *un, *st, step, s);
-  launch_uneven<true>(*op, *un, *st, step, s);
+  launch_uneven<false>(*op, *un, *st, step, live, s);
+  launch_uneven<true>(*op, *un, *st, step, live, s);
   if (st->em)
-    hipLaunchKernelGGL(em_uneven_kernel, dim3((unsigned)op->K), dim3(256), 0, s, *op, *un, *st,
+    hipLaunchKernelGGL(em_uneven_kernel, dim3((unsigned)live), dim3(256), 0, s, *op, *un, *st,
                        *st->em, step);
   return check_hip(hipGetLastError(), "prh_uneven_step launch");
 }

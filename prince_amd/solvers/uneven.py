@@ -75,9 +75,13 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
     given).
     """
 
-    #: right-hand sides per wave of the uneven stage kernel (2: 2.18 vs 2.26 s
-    #: for the first 1024 steps of K = 1024 columns with 4)
+    #: right-hand sides per wave of the uneven stage kernel
     kt_uneven = 2
+    #: value slots per column: 2 lets the windows that start within a group of
+    #: steps be folded together before the group (one pass over the fold
+    #: kernel per group instead of per step); 1 folds each window in place at
+    #: its first step
+    value_slots = 2
 
     def __new__(cls, *args, **kwargs):
         return object.__new__(cls)
@@ -106,6 +110,22 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
             cols.append((zg, np.asarray(win), np.asarray(zs)))
         order = np.argsort([-(len(c[0]) - 1) for c in cols], kind="stable")
         return cols, order
+
+    @staticmethod
+    def _refold_group(starts, smax=64):
+        """Largest S (a power of two up to smax) such that no column starts two
+        windows within one aligned group of S steps: then the windows starting
+        in a group can be folded before it, each into the slot its
+        predecessor's predecessor has left."""
+        T = starts.shape[0]
+        S = smax
+        while S > 1:
+            pad = (-T) % S
+            grp = np.concatenate([starts, np.zeros((pad, starts.shape[1]), dtype=bool)])
+            if grp.reshape(-1, S, starts.shape[1]).sum(axis=1).max() <= 1:
+                return S
+            S //= 2
+        return 1
 
     def _device_tables(self, zw, sw, dev):
         """Photon vectors [nwin][dph] and pair rates [nwin][d] of all column windows."""
@@ -160,13 +180,22 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
             sw.extend([self.ebl_scale[k]] * len(zs))
             firsts.append(off)
             off += len(zs)
-        # refresh lists: columns whose window changes at step t
+        live = np.count_nonzero(gwin >= 0, axis=1).astype(np.int32)
+        if not all(np.all(gwin[t, :live[t]] >= 0) for t in range(T)):
+            raise AssertionError("live columns must lead (columns ordered by decreasing steps)")
+        # re-fold lists: the windows starting at step t, or with two value
+        # slots all windows starting in the group of S steps from t
+        starts = np.zeros((T, K), dtype=bool)
+        starts[0] = gwin[0] >= 0
+        starts[1:] = (gwin[1:] >= 0) & (gwin[1:] != gwin[:-1])
+        nslot = 2 if self.value_slots == 2 else 1
+        S = self._refold_group(starts) if nslot == 2 else 1
         for t in range(T):
-            prev = gwin[t - 1] if t > 0 else np.full(K, -1)
-            cur = gwin[t]
-            chg = np.flatnonzero((cur >= 0) & (cur != prev))
-            ref_in.extend(cur[chg].tolist())
-            ref_out.extend(chg.tolist())
+            if t % S == 0:
+                tt, cc = np.nonzero(starts[t:t + S])
+                w = gwin[t + tt, cc]
+                ref_in.extend(w.tolist())
+                ref_out.extend(((w & 1) * K + cc if nslot == 2 else cc).tolist())
             ref_off.append(len(ref_in))
         zw = np.asarray(zw)
         sw = np.asarray(sw)
@@ -208,7 +237,7 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
             "tasks": dv(tasks, np.int32),
             "blocks": dv(wl.blocks if wl.blocks.size else np.zeros((1, 4)), np.int32),
             "B_sel": lay["B_sel"], "phot": phot, "rpair": rpair,
-            "mrhs": torch.zeros((K, stride), dtype=torch.float64, device=dev),
+            "mrhs": torch.zeros((nslot * K, stride), dtype=torch.float64, device=dev),
             "egrid": dv(run.cr_grid.grid, np.float64), "dcoef": dv(dcoef, np.float64),
             "ddiag": dv(ddiag, np.float64), "pscale": dv(pscale, np.float64),
             "src_slot": dv(src_slot, np.int32), "src_grid": dv(src_grid, np.float64),
@@ -234,7 +263,8 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
             nsteps=T, nwin=int(zw.size), h=bufs["h"].data_ptr(), dldz=bufs["dldz"].data_ptr(),
             hadia=bufs["hadia"].data_ptr(), finj=bufs["finj"].data_ptr(),
             win=bufs["win"].data_ptr(), src_coef=bufs["src_coef"].data_ptr(),
-            rpair=bufs["rpair"].data_ptr(), mrhs=bufs["mrhs"].data_ptr(),
+            rpair=bufs["rpair"].data_ptr(), mrhs=bufs["mrhs"].data_ptr(), nslot=nslot,
+            live=live.ctypes.data,
         )
         st = _lib.PrhState(x=x.data_ptr(), a=bufs["a"].data_ptr(), fphi=bufs["fphi"].data_ptr(),
                            step_ctl=bufs["step_ctl"].data_ptr())
@@ -249,6 +279,7 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
             "ref_off": np.asarray(ref_off), "stride": stride, "dph": cache.dph, "K": K,
             "stream": torch.cuda.current_stream(dev).cuda_stream, "layout": wl, "kt": kt,
             "nwin": int(zw.size), "col_steps": nsteps, "graph": None, "em_bufs": em_bufs,
+            "live": live, "refold_group": S, "nslot": nslot,
             "em": em_struct, "two": False, "wide": False,
         }
         self.timings["setup_host_s"] = time.perf_counter() - t0

@@ -116,3 +116,31 @@ def test_uneven_batch_with_em_cascade_matches_column_solves():
         run.photon_field = base
     assert not np.any(g.state[sl[11]]) and not np.any(g.state[sl[-11]])
     assert worst_n <= TOL and worst_g <= TOL
+
+
+def test_uneven_value_slots_are_bitwise_neutral():
+    """Folding each window in place at its first step (one value slot) and
+    folding the windows of a step group ahead into two slots give the same
+    bits: every column's arithmetic is the same."""
+    run = workloads.nuclear_run()
+    K = 7
+    rng = np.random.default_rng(5)
+    z_max = rng.uniform(0.05, 0.16, size=K)
+    scale = rng.uniform(0.6, 1.4, size=K)
+    srcs = workloads.batch_sources(run, K, seed=9)
+    states = {}
+    for slots in (1, 2):
+        g = UnevenBatchSolverETD2(z_max, run, ebl_scale=scale)
+        g.value_slots = slots
+        for k in range(K):
+            g[k].add_source_class(srcs[k])
+        g.solve(dz=1e-3)
+        plan = g._plan
+        assert plan["nslot"] == slots
+        assert (plan["refold_group"] == 1) == (slots == 1)
+        live = plan["live"]
+        assert live[0] == K and np.all(np.diff(live) <= 0) and live[-1] >= 1
+        print(f"slots {slots}: re-fold group {plan['refold_group']} steps, "
+              f"live columns {live[0]} -> {live[-1]}")
+        states[slots] = g.state.copy()
+    assert np.array_equal(states[1], states[2])
