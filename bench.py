@@ -144,10 +144,6 @@ def _cfg_str(cfg):
 def stage_kernel_names(plan):
     """Per stage (1, 2): the names (as rocprofv3 prints them) of the kernels
     one stage launch of the plan runs."""
-    if plan.get("wide"):
-        comb = plan.get("wheavy") is not None and len(plan["wheavy"]) > 0
-        return [[f"etd2_wide<{s2}>"] + ([f"etd2_wide_combine<{s2}>"] if comb else [])
-                for s2 in ("false", "true")]
     kt, two = plan["kt"], "true" if plan["two"] else "false"
     wgw = plan["op"].wg_waves or 16
     dm = {None: 0, "pass": 1, "chunks": 2}[plan.get("dense_mode")]
@@ -609,12 +605,10 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(args.traffic, stage_kernel_names(bplan)),
-            "kernel": (("etd2_wide + etd2_wide_combine" if bplan.get("wide") else "etd2_stage")
-                       + " (stage 1 and 2, batched K=%d)" % K),
+            "kernel": "etd2_stage (stage 1 and 2, batched K=%d)" % K,
             "bytes_per_launch": nbytes,
             "avg_launch_us": round(avg * 1e6, 2),
         }
-        kernels["stage_layout"] = "wide" if bplan.get("wide") else "narrow"
         kernels["etd2_stage1_us"] = round(t1s * 1e6, 2)
         kernels["etd2_stage2_us"] = round(t2s * 1e6, 2)
         # fold GEMM (MFMA) timing

@@ -71,23 +71,11 @@ typedef struct prh_operator {
   const double* mvals_e;   /* [nwin][mvals_stride] second component or NULL */
   const double* rpair_e;   /* [nwin][d] second component's pair rate or NULL */
   const double* escale;    /* [K] second component's scale per rhs or NULL */
-  /* Wide layout (wide = 1): state [nspec][d][K] with lanes = right-hand
-   * sides, K % 64 == 0, d == 64, one photon field, no step recording;
-   * src_grid is then [nslot][J][d][K].  wtasks [nwtasks][5] = {da,
-   * blk_begin, blk_end, slot, kind}: slot >= 0 stores the part's row sums at
-   * wpart[slot]; wheavy [nheavy][3] = {da, first slot, nparts} lists the rows
-   * whose parts are summed (in part order) and finished by a second kernel. */
-  int32_t wide;
-  int32_t nwtasks;
-  const int32_t* wtasks;
-  int32_t nheavy;
-  int32_t pad2_;
-  const int32_t* wheavy;
-  /* Waves per workgroup of the narrow stage kernels: 0 or 16 (default), or 8
+  /* Waves per workgroup of the stage kernels: 0 or 16 (default), or 8
    * (better at large K).  tasks are packed for it (ntasks a multiple); the
    * adjoint and uneven kernels take 16 only. */
   int32_t wg_waves;
-  /* Dense rows (narrow one-field stages only; ndense = 0: off).  Their blocks
+  /* Dense rows (one-field stages only; ndense = 0: off).  Their blocks
    * from the listed mothers are applied by a mother-major pass over chunks of
    * dchunk species (ndchunk chunks): dense_tab [ndense][nspec][3] = {o_lo,
    * nslot (0: no block), value offset}; dense_row [nspec] = dense index or -1;
@@ -228,7 +216,6 @@ typedef struct prh_state {
   double* rec_fa;
   int32_t rec_first;
   int32_t rec_count;
-  double* wpart;     /* [nslots][d][K] partial row sums (wide layout) or NULL */
   const prh_em* em;  /* host pointer: EM sector stepped after stage 2, or NULL */
   double* dpart;     /* [ndchunk][ndense][K][d] dense-row chunk partials, or NULL */
 } prh_state;
@@ -343,7 +330,7 @@ int32_t prh_fold_rows(const double* kern, int64_t nrows, int32_t dph, const doub
                       const int32_t* in_rows, const int32_t* out_rows, int32_t ncol, double* out,
                       int64_t out_stride, void* stream);
 
-/* Both ETD2 stages of step `step` of the uneven batch (narrow layout, kt 1/2/4).
+/* Both ETD2 stages of step `step` of the uneven batch (kt 1/2/4).
  * Each column follows its own reference solve (propagation.py:813-890 with
  * etd2.py:412-439); only the un->live[step] leading columns are launched.  With st->em set, the EM
  * step of every live column follows (its own window's T_gamma, T_electron,

@@ -76,12 +76,6 @@ class PrhOperator(ctypes.Structure):
         ("mvals_e", _p),
         ("rpair_e", _p),
         ("escale", _p),
-        ("wide", _i32),
-        ("nwtasks", _i32),
-        ("wtasks", _p),
-        ("nheavy", _i32),
-        ("pad2_", _i32),
-        ("wheavy", _p),
         ("wg_waves", _i32),
         ("ndense", _i32),
         ("dchunk", _i32),
@@ -119,7 +113,6 @@ class PrhState(ctypes.Structure):
         ("rec_fa", _p),
         ("rec_first", _i32),
         ("rec_count", _i32),
-        ("wpart", _p),
         ("em", _p),
         ("dpart", _p),
     ]

@@ -1069,20 +1069,10 @@ __global__ void step_ctl_advance(int32_t* ctl, int32_t chunk) {
   if (threadIdx.x == 0 && blockIdx.x == 0) ctl[0] = ctl[0] + chunk;
 }
 
-// ---------------------------------------------------------------------------
-// Wide stage kernels: lanes = right-hand sides (K a multiple of 64), state
-// layout [species][bin][rhs] -- the reference's multi-RHS (n, K) row-major
-// order.  One wave per (row part, 64-rhs group).  Coupling values are
-// wave-uniform (scalar loads); every block issues 64 independent coalesced
-// state loads; the FD stencil runs in registers over the 64 bins.  Parts of
-// long rows store their partial row sums; etd2_wide_combine adds them in part
-// order and finishes those rows.
-// ---------------------------------------------------------------------------
-constexpr int WTHREADS = 256;
-constexpr int WWAVES = WTHREADS / WAVE;
-
-__device__ __forceinline__ void wide_step(const prh_state& st, int step_local, int use_ctl,
-                                          int& step, bool& live) {
+// Step of a launch: step_local itself, or step_ctl[0] + step_local under graph
+// replay (use_ctl), where steps at or beyond step_ctl[1] do nothing.
+__device__ __forceinline__ void resolve_step(const prh_state& st, int step_local, int use_ctl,
+                                             int& step, bool& live) {
   step = step_local;
   live = true;
   if (use_ctl) {
@@ -1091,319 +1081,22 @@ __device__ __forceinline__ void wide_step(const prh_state& st, int step_local, i
   }
 }
 
-// wave -> (task index, rhs group); XCD-aware when the group count is a multiple of 8
-__device__ __forceinline__ void wide_coords(int ntask, int ngroup, int& task, int& g) {
-  const int wv = wave_id();
-  const int nwg_t = (ntask + WWAVES - 1) / WWAVES;
-  const int L = blockIdx.x;
-  int tw;
-  if (ngroup % 8 == 0) {
-    const int j = L >> 3;
-    g = (L & 7) + 8 * (j / nwg_t);
-    tw = j % nwg_t;
-  } else {
-    g = L / nwg_t;
-    tw = L % nwg_t;
-  }
-  // wave-uniform by construction; readfirstlane lets the compiler keep every
-  // task, block descriptor and coupling value in scalar registers / loads
-  task = __builtin_amdgcn_readfirstlane(tw * WWAVES + wv);
-  g = __builtin_amdgcn_readfirstlane(g);
-}
-
-// Buffer access to one (species, 64-rhs group) tile: element (bin i, lane) at
-// byte offset i * K * 8 (scalar soffset) + lane * 8 (vector voffset) from a
-// uniform base, so no per-lane 64-bit address arithmetic is needed.
-
-template <bool STAGE2>
-__device__ __forceinline__ void wide_epilogue(const prh_operator& op, const prh_schedule& sch,
-                                              const prh_state& st, int step, int w, int da,
-                                              int g, double (&acc)[64], double* lds) {
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int K = op.K;
-  const int K8 = K * 8;
-  const int v8 = lane * 8;
-  const int k = g * WAVE + lane;
-  const bool pdiff = (op.flags & PRH_F_PARTIAL_DIFF) != 0;
-  const double hh = sch.h[step];
-  const double dl = sch.dldz[step];
-  {
-    // lane-parallel coefficients of bin i = lane, shared through LDS
-    const RowIn ri = row_in<false>(op, sch, step, w, da, lane);
-    const RowCoef rc = row_coef_from<false>(ri, 0.0, pdiff);
-    const double hd = rc.dval * hh;
-    const double e = exp(hd);
-    lds[lane] = rc.kap;
-    lds[64 + lane] = e;
-    lds[128 + lane] = STAGE2 ? phi2_of(hd, e) : phi1_of(hd, e);
-  }
-  __builtin_amdgcn_wave_barrier();
-  const int64_t rowbase = (int64_t)da * 64 * K + (int64_t)g * WAVE;
-  const int tile = 64 * K8;
-  const auto rx = tile_rsrc((STAGE2 ? st.a : st.x) + rowbase, tile);
-  const auto ra = tile_rsrc(st.a + rowbase, tile);
-  const auto rf = tile_rsrc(st.fphi + rowbase, tile);
-  const auto rxo = tile_rsrc(st.x + rowbase, tile);
-  const bool inj = (op.flags & PRH_F_INJECTION) != 0 && op.src_slot[da] >= 0;
-  const int slot = inj ? op.src_slot[da] : 0;
-  const int J = op.nsrc < 4 ? op.nsrc : 4;
-  const double finj = sch.finj[step];
-  double coef[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j)
-    coef[j] = (inj && j < J) ? sch.src_coef[((int64_t)step * op.nsrc + j) * K + k] : 0.0;
-  const auto rs = tile_rsrc(op.src_grid + (int64_t)slot * op.nsrc * 64 * K + (int64_t)g * WAVE,
-                            (J > 0 ? J : 1) * tile);
-  // sliding window of x and kappa * x over bins i-3 .. i+3
-  double xw[7], kw[7];
-#pragma unroll
-  for (int q = 0; q < 7; ++q) {
-    const int j = q - 3;
-    xw[q] = (j >= 0 && j < 64) ? tload(rx, v8, j * K8) : 0.0;
-    kw[q] = (j >= 0 && j < 64) ? lds[j] * xw[q] : 0.0;
-  }
-#pragma unroll
-  for (int i = 0; i < 64; ++i) {
-    double out = acc[i];
-    if (pdiff) {
-      double dx = 0.0;
-#pragma unroll
-      for (int o = -3; o <= 3; ++o) {
-        if (o == 0 || i + o < 0 || i + o >= 64) continue;
-        const double c = op.dcoef[i * 7 + o + 3];
-        if (c != 0.0) dx += c * kw[o + 3];
-      }
-      out = out + dx;
-    }
-    out = out * dl;
-    if (inj) {
-      double bsum = 0.0;
-      for (int j = 0; j < J; ++j) {
-        const double gv = tload(rs, v8, (j * 64 + i) * K8);
-        bsum = (j == 0) ? coef[j] * gv : bsum + coef[j] * gv;
-      }
-      out = out + finj * bsum;
-    }
-    const double xi = xw[3];
-    const double e = lds[64 + i];
-    const double phi = lds[128 + i];
-    if (!STAGE2) {
-      double a = e * xi;
-      double sc = phi * out;
-      sc = sc * hh;
-      a = a + sc;
-      tstore(a, ra, v8, i * K8);
-      tstore(out, rf, v8, i * K8);
-    } else {
-      double sc = out - tload(rf, v8, i * K8);
-      sc = sc * hh;
-      sc = sc * phi;
-      tstore(xi + sc, rxo, v8, i * K8);
-    }
-    // slide the window by one bin
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      xw[q] = xw[q + 1];
-      kw[q] = kw[q + 1];
-    }
-    const int jn = i + 4;
-    xw[6] = (jn < 64) ? tload(rx, v8, jn * K8) : 0.0;
-    kw[6] = (jn < 64) ? lds[jn] * xw[6] : 0.0;
-  }
-}
-
-// Band block of NS diagonals starting at offset OLO:
-// acc[i] += sum_{s < NS} m[s * 64 + i] * x[i + OLO + s], each x row loaded once.
-template <int OLO, int NS>
-__device__ __forceinline__ void wide_band(__amdgpu_buffer_rsrc_t r, const double* __restrict__ m,
-                                          int K8, int v8, double (&acc)[64]) {
-  constexpr int JEND = 64 - OLO;  // rows jx = jj + OLO < 64
-#pragma unroll
-  for (int j0 = 0; j0 < JEND; j0 += 8) {
-    double xv[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      xv[u] = (j0 + u < JEND) ? tload(r, v8, (j0 + u + OLO) * K8) : 0.0;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int jj = j0 + u;
-#pragma unroll
-      for (int s2 = 0; s2 < NS; ++s2) {
-        const int i = jj - s2;
-        if (jj < JEND && i >= 0 && i < 64) acc[i] += m[s2 * 64 + i] * xv[u];
-      }
-    }
-  }
-}
-
-// One block into acc.  The host admits only diagonal (0, 1) and the band
-// shapes (0, 8) and (1, 7) to this layout.
-__device__ __forceinline__ void wide_block(__amdgpu_buffer_rsrc_t r, const double* __restrict__ m,
-                                           int olo, int ns, int K8, int v8, double (&acc)[64]) {
-  if (ns == 1) {
-#pragma unroll
-    for (int i0 = 0; i0 < 64; i0 += 16) {
-      double xv[16];
-#pragma unroll
-      for (int u = 0; u < 16; ++u) xv[u] = tload(r, v8, (i0 + u) * K8);
-#pragma unroll
-      for (int u = 0; u < 16; ++u) acc[i0 + u] += m[i0 + u] * xv[u];
-    }
-  } else if (olo == 0) {
-    wide_band<0, 8>(r, m, K8, v8, acc);
-  } else {
-    wide_band<1, 7>(r, m, K8, v8, acc);
-  }
-}
-
-template <bool STAGE2>
-__global__ void __launch_bounds__(WTHREADS) etd2_wide(prh_operator op, prh_schedule sch,
-                                                      prh_state st, int step_local, int use_ctl) {
-  __shared__ double lds_all[WWAVES][3 * 64];
-  int step;
-  bool live;
-  wide_step(st, step_local, use_ctl, step, live);
-  if (!live) return;
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int K = op.K;
-  int t, g;
-  wide_coords(op.nwtasks, K / 64, t, g);
-  if (t >= op.nwtasks) return;
-  const int* tk = op.wtasks + 5 * t;
-  const int da = tk[0];
-  const int b_begin = tk[1];
-  const int b_end = tk[2];
-  const int slot = tk[3];
-  const double* __restrict__ xg = (STAGE2 ? st.a : st.x) + (int64_t)g * WAVE;
-  const int w = sch.win[step];
-  const double* __restrict__ mv = op.mvals + (int64_t)w * op.mvals_stride;
-  double acc[64];
-#pragma unroll
-  for (int i = 0; i < 64; ++i) acc[i] = 0.0;
-  const int K8 = K * 8;
-  const int v8 = lane * 8;
-  for (int b = b_begin; b < b_end; ++b) {
-    const int* blk = op.blocks + 4 * b;
-    const auto r = tile_rsrc(xg + (int64_t)blk[0] * 64 * K, 64 * K8);
-    wide_block(r, mv + blk[3], blk[1], blk[2], K8, v8, acc);
-  }
-  if (slot >= 0) {
-    const auto rp = tile_rsrc(st.wpart + (int64_t)slot * 64 * K + (int64_t)g * WAVE, 64 * K8);
-#pragma unroll
-    for (int i = 0; i < 64; ++i) tstore(acc[i], rp, v8, i * K8);
-    return;
-  }
-  wide_epilogue<STAGE2>(op, sch, st, step, w, da, g, acc, lds_all[threadIdx.x / WAVE]);
-}
-
-template <bool STAGE2>
-__global__ void __launch_bounds__(WTHREADS) etd2_wide_combine(prh_operator op, prh_schedule sch,
-                                                              prh_state st, int step_local,
-                                                              int use_ctl) {
-  __shared__ double lds_all[WWAVES][3 * 64];
-  int step;
-  bool live;
-  wide_step(st, step_local, use_ctl, step, live);
-  if (!live) return;
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int K = op.K;
-  int t, g;
-  wide_coords(op.nheavy, K / 64, t, g);
-  if (t >= op.nheavy) return;
-  const int* hv = op.wheavy + 3 * t;
-  const int da = hv[0];
-  const int slot0 = hv[1];
-  const int np = hv[2];
-  const int K8 = K * 8;
-  const int v8 = lane * 8;
-  const double* pp = st.wpart + (int64_t)slot0 * 64 * K + (int64_t)g * WAVE;
-  double acc[64];
-  {
-    const auto r0 = tile_rsrc(pp, 64 * K8);
-#pragma unroll
-    for (int i = 0; i < 64; ++i) acc[i] = tload(r0, v8, i * K8);
-  }
-  for (int p = 1; p < np; ++p) {
-    const auto rq = tile_rsrc(pp + (int64_t)p * 64 * K, 64 * K8);
-#pragma unroll
-    for (int i = 0; i < 64; ++i) acc[i] += tload(rq, v8, i * K8);
-  }
-  const int w = sch.win[step];
-  wide_epilogue<STAGE2>(op, sch, st, step, w, da, g, acc, lds_all[threadIdx.x / WAVE]);
-}
-
-template <bool S2>
-void launch_wide(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
-                 int step_local, int use_ctl, hipStream_t s) {
-  const int ng = op.K / 64;
-  const int64_t nwg = (int64_t)((op.nwtasks + WWAVES - 1) / WWAVES) * ng;
-  hipLaunchKernelGGL((etd2_wide<S2>), dim3((unsigned)nwg), dim3(WTHREADS), 0, s, op, sch, st,
-                     step_local, use_ctl);
-  if (op.nheavy > 0) {
-    const int64_t nwh = (int64_t)((op.nheavy + WWAVES - 1) / WWAVES) * ng;
-    hipLaunchKernelGGL((etd2_wide_combine<S2>), dim3((unsigned)nwh), dim3(WTHREADS), 0, s, op,
-                       sch, st, step_local, use_ctl);
-  }
-}
-
 // ---------------------------------------------------------------------------
-// EM sector (photons, e+-): one launch pair per step after stage 2.
-// em_bh_weights: w[j][k] = bhw[win][j] * sum_s cbh[s] x[s][j][k]; one wave per
-// (bin j, 64-rhs group), species summed in state order.
-// em_step: one 1024-thread workgroup per 64-rhs group (lane = rhs); the photon
-// column lives in LDS ([bin][lane], conflict-free); wave v owns bins v + 16 t.
+// EM sector (photons, e+-): three launches per step after stage 2 -- the
+// Bethe-Heitler pair weights, the photon ETD2 step and the cascade transfer.
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(256) em_bh_weights(prh_operator op, prh_schedule sch,
-                                                     prh_state st, prh_em em, int64_t sS,
-                                                     int64_t sJ, int64_t sK, int step_local,
-                                                     int use_ctl) {
-  // workgroup = (bin j, 64-rhs group); the 4 waves sum interleaved quarters of
-  // the species (8 loads in flight each) and combine them in wave order
-  __shared__ double part[4][WAVE];
-  int step;
-  bool live;
-  wide_step(st, step_local, use_ctl, step, live);
-  if (!live) return;
-  const int lane = threadIdx.x & (WAVE - 1);
-  const int wv = wave_id();
-  const int K = op.K;
-  const int d = op.d;
-  const int ng = (K + WAVE - 1) / WAVE;
-  const int j = blockIdx.x / ng;
-  const int k = (blockIdx.x % ng) * WAVE + lane;
-  const bool act = k < K;
-  const double* xp = st.x + j * sJ + (act ? k : 0) * sK;
-  const int ns = op.nspec;
-  double acc = 0.0;
-  int s = wv;
-  for (; s + 28 < ns; s += 32) {
-    double xv[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) xv[u] = act ? xp[(int64_t)(s + 4 * u) * sS] : 0.0;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) acc += em.cbh[s + 4 * u] * xv[u];
-  }
-  for (; s < ns; s += 4) acc += em.cbh[s] * (act ? xp[(int64_t)s * sS] : 0.0);
-  part[wv][lane] = acc;
-  __syncthreads();
-  if (wv != 0 || !act) return;
-  const double tot = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
-  const int w = sch.win[step];
-  em.wbh[(int64_t)j * K + k] = em.bhw[(int64_t)w * d + j] * tot;
-}
-
-// Narrow layout ([species][rhs][bin]): workgroup = one right-hand side, lanes
-// = bins, so every species panel is one coalesced 512-byte load; the 4 waves
-// take the species in the same interleaved quarters and order as
-// em_bh_weights and combine them in wave order (identical sums).
+// Bethe-Heitler weights w[j][k] = bhw[win][j] * sum_s cbh[s] x[s][k][j]:
+// workgroup = one right-hand side, lanes = bins, so every species panel is one
+// coalesced 512-byte load; the 4 waves sum interleaved quarters of the species
+// (8 loads in flight each) and combine them in wave order.
 __global__ void __launch_bounds__(256) em_bh_weights_bins(prh_operator op, prh_schedule sch,
                                                           prh_state st, prh_em em,
                                                           int step_local, int use_ctl) {
   __shared__ double part[4][WAVE];
   int step;
   bool live;
-  wide_step(st, step_local, use_ctl, step, live);
+  resolve_step(st, step_local, use_ctl, step, live);
   if (!live) return;
   const int lane = threadIdx.x & (WAVE - 1);
   const int wv = wave_id();
@@ -1443,7 +1136,7 @@ __global__ void __launch_bounds__(256) em_photon_kernel(prh_operator op, prh_sch
   __shared__ double ga[PRH_EM_MAX];
   int step;
   bool live;
-  wide_step(st, step_local, use_ctl, step, live);
+  resolve_step(st, step_local, use_ctl, step, live);
   if (!live) return;
   const int K = op.K;
   const int de = em.d_em;
@@ -1505,7 +1198,7 @@ __global__ void __launch_bounds__(256) em_transfer_mfma(prh_operator op, prh_sch
                                                        int use_ctl) {
   int step;
   bool live;
-  wide_step(st, step_local, use_ctl, step, live);
+  resolve_step(st, step_local, use_ctl, step, live);
   if (!live) return;
   const int K = op.K;
   const int de = em.d_em;
@@ -1574,24 +1267,8 @@ __global__ void em_stack_kernel(const double* __restrict__ tg, const double* __r
 void launch_em(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
                int step_local, int use_ctl, hipStream_t s) {
   const prh_em& em = *st.em;
-  const int64_t K = op.K, d = op.d;
-  int64_t sS, sJ, sK;
-  if (op.wide) {
-    sS = d * K;
-    sJ = K;
-    sK = 1;
-  } else {
-    sS = K * d;
-    sJ = 1;
-    sK = d;
-  }
-  const int ng = (op.K + WAVE - 1) / WAVE;
-  if (op.wide)
-    hipLaunchKernelGGL(em_bh_weights, dim3((unsigned)(op.d * ng)), dim3(256), 0, s, op, sch, st,
-                       em, sS, sJ, sK, step_local, use_ctl);
-  else
-    hipLaunchKernelGGL(em_bh_weights_bins, dim3((unsigned)op.K), dim3(256), 0, s, op, sch, st, em,
-                       step_local, use_ctl);
+  hipLaunchKernelGGL(em_bh_weights_bins, dim3((unsigned)op.K), dim3(256), 0, s, op, sch, st, em,
+                     step_local, use_ctl);
   hipLaunchKernelGGL(em_photon_kernel, dim3((unsigned)op.K), dim3(192), 0, s, op, sch, st, em,
                      step_local, use_ctl);
   const int64_t tiles = (int64_t)((em.d_em + 15) / 16) * ((op.K + 15) / 16);
@@ -1608,7 +1285,7 @@ int32_t check_uneven(const prh_operator* op, const prh_uneven* un, const prh_sta
   if (op->ntasks <= 0 || op->ntasks % SWG != 0)
     return set_error(w + ": ntasks must be a positive multiple of 16");
   if (!(op->kt == 1 || op->kt == 2 || op->kt == 4)) return set_error(w + ": kt must be 1, 2 or 4");
-  if (op->wide || op->mvals_e) return set_error(w + ": narrow one-field operators only");
+  if (op->mvals_e) return set_error(w + ": one-field operators only");
   if (op->wg_waves == 8) return set_error(w + ": the uneven kernel runs 16-wave workgroups");
   if (op->ndense > 0) return set_error(w + ": the uneven kernel has no dense-row pass");
   if (!un->h || !un->dldz || !un->hadia || !un->finj || !un->win || !un->rpair || !un->mrhs)
@@ -1722,12 +1399,14 @@ __global__ void __launch_bounds__(THREADS) fold_rows_mfma(const double* __restri
 // Coupling product of one row part for KT columns of the uneven batch, each
 // with its own window values (vbase[kk]: the column's value row in mrhs):
 //   acc[kk] = sum_b m_{b,k} (shifted) y_b[k]
-// in block order, as couple_part.  Runs of diagonal blocks and of band blocks
-// issue the loads of UB blocks before any product, through buffer resources
-// (one per column's value row, one per mother panel).
+// in block order, as couple_part.  Runs of diagonal blocks issue the loads of
+// DIAG blocks before any product, a band block those of half its diagonals,
+// through buffer resources (one per column's value row, one per mother panel).
+// Two diagonal blocks per batch at KT = 2: 4.87 s for the K = 1024 march
+// against 4.93 s with six.
 template <int KT>
 struct UnevenBatch {
-  static constexpr int DIAG = KT == 1 ? 8 : (KT == 2 ? 6 : 3);
+  static constexpr int DIAG = KT == 1 ? 4 : (KT == 2 ? 2 : 1);
 };
 
 template <int KT>
@@ -1874,12 +1553,12 @@ __device__ __forceinline__ void couple_cols(const prh_operator& op, const double
 
 // One stage of the uneven batch for the live columns [0, ncols) of step
 // `step`.  Workgroup L runs on XCD L % 8; XCD x takes the column groups x,
-// x + 8, ... and walks all row groups of one column group before the next, so
-// the mother panels of the few column groups in flight stay in its L2 (the
-// coupling values are read once per stage whatever the order).
+// x + 8, ... and walks them for one row group before the next (4.87 vs 5.37 s
+// for the K = 1024 march with row groups walked first; the coupling values are
+// read once per stage either way).
 template <bool STAGE2, int KT>
-__global__ void __launch_bounds__(STHREADS, KT <= 2 ? 8 : 4) etd2_uneven(prh_operator op, prh_uneven un,
-                                                        prh_state st, int step, int ncols) {
+__global__ void __launch_bounds__(STHREADS, KT <= 2 ? 8 : 4)
+    etd2_uneven(prh_operator op, prh_uneven un, prh_state st, int step, int ncols) {
   __shared__ double red[SWG][KT][WAVE];
   const int lane = threadIdx.x & (WAVE - 1);
   const int wv = wave_id();
@@ -1889,8 +1568,9 @@ __global__ void __launch_bounds__(STHREADS, KT <= 2 ? 8 : 4) etd2_uneven(prh_ope
   const int ny = (ncols + KT - 1) / KT;
   const int L = blockIdx.x;
   const int j = L >> 3;
-  const int gy = (L & 7) + 8 * (j / nx);
-  const int gx = j % nx;
+  const int per = (ny + 7) / 8;
+  const int gy = (L & 7) + 8 * (j % per);
+  const int gx = j / per;
   if (gy >= ny) return;  // padding of the column groups to a multiple of 8
   const int k0 = gy * KT;
   const int kend = ncols;
@@ -2234,8 +1914,7 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
     return set_error(std::string(who) + ": ntasks must be a multiple of the workgroup waves");
   if (wg_waves_of(*op) != 16 && std::string(who).rfind("prh_adjoint", 0) == 0)
     return set_error(std::string(who) + ": the adjoint kernels run 16-wave workgroups");
-  if ((op->wide ? op->nwtasks : op->ntasks) <= 0)
-    return set_error(std::string(who) + ": empty operator");
+  if (op->ntasks <= 0) return set_error(std::string(who) + ": empty operator");
   if (!(op->kt == 1 || op->kt == 2 || op->kt == 4 || op->kt == 8 || op->kt == 16))
     return set_error(std::string(who) + ": kt must be 1, 2, 4, 8 or 16");
   if (op->kt == 16 && (op->mvals_e || std::string(who).rfind("prh_adjoint", 0) == 0 || st->rec_x))
@@ -2244,18 +1923,6 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
     return set_error(std::string(who) + ": grid too large");
   if (op->mvals_stride * 8 > 0x7fffffff || (int64_t)op->K * op->d * 8 > 0x7fffffff)
     return set_error(std::string(who) + ": a window's values or one species' [K][d] panel exceeds 2 GiB");
-  if (op->wide) {
-    if (op->K % 64 != 0 || op->d != 64)
-      return set_error(std::string(who) + ": the wide layout needs K % 64 == 0 and d == 64");
-    if ((int64_t)op->K * 64 * 8 * (op->nsrc > 1 ? op->nsrc : 1) >= ((int64_t)1 << 31))
-      return set_error(std::string(who) + ": K too large for the wide layout's 32-bit tile offsets");
-    if (op->mvals_e) return set_error(std::string(who) + ": the wide layout has one photon field");
-    if (!op->wtasks) return set_error(std::string(who) + ": no wide tasks");
-    if (op->nheavy > 0 && (!op->wheavy || !st->wpart))
-      return set_error(std::string(who) + ": split rows need wheavy and the partial buffer");
-    if (op->nsrc > 4) return set_error(std::string(who) + ": the wide layout takes at most 4 sources per rhs");
-    if (st->rec_x) return set_error(std::string(who) + ": the wide layout does not record steps");
-  }
   if (op->nfin != 0) {
     if (op->nfin < 0 || op->nfin % wg_waves_of(*op) != 0 || !op->fin_tasks)
       return set_error(std::string(who) + ": nfin must be a multiple of the workgroup waves, with fin_tasks");
@@ -2265,8 +1932,8 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
   if (op->ndense > 0) {
     if (op->nfin == 0 && op->ndense > DENSE_MAX)
       return set_error(std::string(who) + ": at most 2 dense rows for the mother-major pass");
-    if (op->wide || op->mvals_e || op->kt > 8)
-      return set_error(std::string(who) + ": dense rows are for the narrow one-field stages (kt <= 8)");
+    if (op->mvals_e || op->kt > 8)
+      return set_error(std::string(who) + ": dense rows are for the one-field stages (kt <= 8)");
     if (!op->dense_row || !st->dpart || op->ndchunk < 1)
       return set_error(std::string(who) + ": dense rows need dense_row, dpart and ndchunk >= 1");
     if (op->nfin == 0 && (!op->dense_tab || op->dchunk < 1 ||
@@ -2366,10 +2033,6 @@ void launch_stage_two(const prh_operator& op, const prh_schedule& sch, const prh
 template <bool S2>
 void launch_stage_kt(const prh_operator& op, const prh_schedule& sch, const prh_state& st,
                      int step_local, int use_ctl, hipStream_t s) {
-  if (op.wide) {
-    launch_wide<S2>(op, sch, st, step_local, use_ctl, s);
-    return;
-  }
   if (op.mvals_e) {
     launch_stage_two<S2, true>(op, sch, st, step_local, use_ctl, s);
     return;
@@ -2855,7 +2518,6 @@ int32_t prh_etd2_steps(const prh_operator* op, const prh_schedule* sch, const pr
 int32_t prh_etd2_steps_noalloc(const prh_operator* op, const prh_schedule* sch, double* x,
                                int32_t k_begin, int32_t k_end, void* stream) {
   if (!op || !sch || !x) return set_error("prh_etd2_steps_noalloc: null argument");
-  if (op->wide) return set_error("prh_etd2_steps_noalloc: narrow layout only (wide = 0)");
   if (op->nspec <= 0 || op->K <= 0 || op->d <= 0)
     return set_error("prh_etd2_steps_noalloc: empty operator");
   const int64_t panel = (int64_t)op->nspec * op->K * op->d;
@@ -2909,7 +2571,6 @@ int32_t prh_adjoint_seed(const prh_operator* opT, const prh_schedule* sch,
   if (!adj) return set_error("prh_adjoint_seed: null adjoint");
   prh_state dummy{};
   if (int32_t e = validate(opT, sch, &dummy, "prh_adjoint_seed")) return e;
-  if (opT->wide) return set_error("prh_adjoint_seed: the adjoint runs on the narrow layout (wide = 0)");
   if (step < 0 || step >= sch->nsteps) return set_error("prh_adjoint_seed: step outside the schedule");
   if (!adj->lam || !adj->u) return set_error("prh_adjoint_seed: lam and u are required");
   hipStream_t s = (hipStream_t)stream;
@@ -2926,7 +2587,6 @@ int32_t prh_adjoint_step(const prh_operator* opT, const prh_schedule* sch, const
   if (!adj) return set_error("prh_adjoint_step: null adjoint");
   prh_state dummy{};
   if (int32_t e = validate(opT, sch, &dummy, "prh_adjoint_step")) return e;
-  if (opT->wide) return set_error("prh_adjoint_step: the adjoint runs on the narrow layout (wide = 0)");
   if (step < 0 || step >= sch->nsteps) return set_error("prh_adjoint_step: step outside the schedule");
   if (!adj->lam || !adj->lam_a || !adj->u || !adj->v)
     return set_error("prh_adjoint_step: lam, lam_a, u and v are required");

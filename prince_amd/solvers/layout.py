@@ -312,58 +312,6 @@ def dense_split(blocks, brow_ptr, nspec, min_blocks=100, max_rows=2, max_slots=8
     return rows, row_of, tab, blocks[keep], brow_rest
 
 
-def wide_block_cost(blocks):
-    """Cost of a block in the wide stage kernel (lanes = right-hand sides):
-    one state load per bin plus one FMA per bin and diagonal."""
-    if blocks.size == 0:
-        return np.zeros(0, dtype=np.int64)
-    return 1 + blocks[:, 2].astype(np.int64)
-
-
-#: block shapes (o_lo, nslot) the wide stage kernel has code paths for
-WIDE_SHAPES = ((0, 1), (0, 8), (1, 7))
-
-
-def wide_shapes_ok(blocks):
-    """True when every block is one of WIDE_SHAPES."""
-    if blocks.size == 0:
-        return True
-    shp = blocks[:, 1].astype(np.int64) * 16 + blocks[:, 2].astype(np.int64)
-    return bool(np.isin(shp, [o * 16 + n for o, n in WIDE_SHAPES]).all())
-
-
-def build_wide_tasks(layout: BlockLayout, part_cost=24):
-    """Tasks of the wide stage kernel: one wave per (task, 64-rhs group).
-
-    Returns ``(wtasks, wheavy, nslots)``: ``wtasks`` int32 [T, 5] = {da,
-    b_begin, b_end, slot, kind}, longest first (workgroups are dispatched in
-    index order); a row costing more than ``part_cost`` is cut into parts
-    whose row sums go to partial-buffer slots ``slot`` (consecutive per row,
-    in block order).  ``wheavy`` int32 [H, 3] = {da, first slot, nparts}
-    lists those rows; ``nslots`` is the partial-buffer depth.
-    """
-    cost = wide_block_cost(layout.blocks)
-    tasks = []
-    heavy = []
-    nslots = 0
-    for da in range(layout.nspec):
-        b0, b1 = int(layout.brow_ptr[da]), int(layout.brow_ptr[da + 1])
-        total = int(cost[b0:b1].sum()) if b1 > b0 else 0
-        nparts = max(1, min(-(-total // part_cost), b1 - b0))
-        if nparts == 1:
-            tasks.append((total, [da, b0, b1, -1, part_kind(layout.blocks, b0, b1)]))
-            continue
-        heavy.append([da, nslots, nparts])
-        for p, (c0, c1) in enumerate(split_row(cost, b0, b1, nparts)):
-            tasks.append((int(cost[c0:c1].sum()),
-                          [da, c0, c1, nslots + p, part_kind(layout.blocks, c0, c1)]))
-        nslots += nparts
-    tasks.sort(key=lambda t: -t[0])
-    wtasks = np.asarray([t[1] for t in tasks], dtype=np.int32).reshape(-1, 5)
-    wheavy = np.asarray(heavy, dtype=np.int32).reshape(-1, 3)
-    return wtasks, wheavy, nslots
-
-
 def transposed_layout(indptr, indices, m0, nspec, d, keep_off=True):
     """Block-band layout of M^T whose selection map points at rows of the
     forward fold kernel (the entry (r, c) of M feeds slot (c, r) of M^T)."""

@@ -35,10 +35,8 @@ from .layout import (
     WindowLayout,
     build_block_layout,
     build_tasks,
-    build_wide_tasks,
     build_chunked_tasks,
     dense_split,
-    wide_shapes_ok,
     fd_tables,
     fd_transposed_tables,
     transposed_layout,
@@ -350,31 +348,6 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
                 return kt
         return 1
 
-    #: stage-kernel layout: "narrow" (lanes = bins, the faster one measured at
-    #: K = 512: 250 vs 360 us per stage), "wide" (lanes = right-hand sides;
-    #: needs K % 64 == 0, d == 64, one photon field, <= 4 sources per column,
-    #: no adjoint) or "auto" (wide whenever those conditions hold)
-    stage_layout = "narrow"
-
-    def wide_part_cost_for(self, K):
-        """Cost units per task of the wide kernel (a block costs 1 + nslot)."""
-        tuned = getattr(self, "_tuned", {}).get(("wide_part_cost", K))
-        return tuned if tuned is not None else 96
-
-    def _use_wide(self, with_adjoint=False):
-        """Whether the solve runs the wide stage kernels (lanes = right-hand sides)."""
-        mode = getattr(self, "_tuned", {}).get("layout", self.stage_layout)
-        K, d = self._K, self.prince_run.cr_grid.d
-        J = max([1] + [len(c) for c in self._column_sources()])
-        ok = (K % 64 == 0 and d == 64 and self._split_fields() is None and not with_adjoint
-              and J <= 4)
-        if mode == "narrow":
-            return False
-        if mode == "wide" and not ok:
-            raise ValueError("the wide stage layout needs K % 64 == 0, d == 64, one photon "
-                             "field, at most 4 sources per column and no adjoint")
-        return ok
-
     def __init__(self, *args, **kwargs):
         super().__init__(*args, **kwargs)
         self._lib = _lib.load()
@@ -403,22 +376,18 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         return np.zeros((n, self._K)) if self._K > 1 else np.zeros(n)
 
     def _coerce_state(self, state):
-        """Host state (dim_states[, K]) to the device layout: [nspec][d][K] for the
-        wide kernels, [nspec][K][d] otherwise."""
+        """Host state (dim_states[, K]) to the device layout [nspec][K][d]."""
         torch = _torch()
         nspec, d = self.spec_man.nspec, self.prince_run.cr_grid.d
         K = self._K
-        self._state_wide = self._use_wide()
         state = np.asarray(state, dtype=np.float64).reshape(-1, K)
         self._em_init = state[self.dim_states:] if self.dim_em else None
-        host = state[: self.dim_states].reshape(nspec, d, K)
-        if not self._state_wide:
-            host = host.transpose(0, 2, 1)
+        host = state[: self.dim_states].reshape(nspec, d, K).transpose(0, 2, 1)
         return torch.from_numpy(np.ascontiguousarray(host)).to(self.device)
 
     def state_view(self, state):
-        """Device state as an [nspec, d, K] view, whichever layout it is stored in."""
-        return state if getattr(self, "_state_wide", False) else state.permute(0, 2, 1)
+        """Device state ([nspec][K][d]) as an [nspec, d, K] view."""
+        return state.permute(0, 2, 1)
 
     def _finalize_state(self, state):
         nspec, d = self.spec_man.nspec, self.prince_run.cr_grid.d
@@ -645,45 +614,33 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         t1 = time.perf_counter()
         if with_adjoint and self.enable_em_cascade:
             raise NotImplementedError("the adjoint sweep does not cover the EM cascade")
-        wide = self._use_wide(with_adjoint)
-        if wide and not wide_shapes_ok(wl.fwd.blocks):
-            if getattr(self, "_tuned", {}).get("layout", self.stage_layout) == "wide":
-                raise ValueError("the operator has block shapes the wide stage kernel lacks")
-            wide = False
-        self._state_wide = wide
-        if wide:
-            tasks = np.zeros((0, 6), dtype=np.int32)
-            wtasks, wheavy, nwslots = build_wide_tasks(wl.fwd, self.wide_part_cost_for(K))
-        else:
-            wgw = self.wg_waves_for(K, with_adjoint)
-            dense = chunked = None
-            split_fields = self._split_fields() is not None
-            mode = self.dense_rows_for(K)
-            if mode and not with_adjoint and not split_fields and self.kt_for(K) <= 8:
-                if mode == "pass":
-                    dense = dense_split(wl.blocks, wl.fwd.brow_ptr, nspec)
-                elif mode == "chunks":
-                    chunked = build_chunked_tasks(
-                        wl.fwd, self.part_cost_for(K), wgw, chunk_cost=self.dense_chunk_cost_for(K),
-                        place=getattr(self, "_tuned", {}).get(("dense_place", K), "mother"))
-                else:
-                    raise ValueError(f"unknown dense-row mode {mode!r}")
-            if dense is not None:
-                from dataclasses import replace as _dc_replace
-
-                rest = _dc_replace(wl.fwd, blocks=dense[3], brow_ptr=dense[4])
-                tasks = build_tasks(rest, self.part_cost_for(K), wgw)
-            elif chunked is not None:
-                tasks = chunked[0]
+        wgw = self.wg_waves_for(K, with_adjoint)
+        dense = chunked = None
+        split_fields = self._split_fields() is not None
+        mode = self.dense_rows_for(K)
+        if mode and not with_adjoint and not split_fields and self.kt_for(K) <= 8:
+            if mode == "pass":
+                dense = dense_split(wl.blocks, wl.fwd.brow_ptr, nspec)
+            elif mode == "chunks":
+                chunked = build_chunked_tasks(
+                    wl.fwd, self.part_cost_for(K), wgw, chunk_cost=self.dense_chunk_cost_for(K),
+                    place=getattr(self, "_tuned", {}).get(("dense_place", K), "mother"))
             else:
-                tasks = build_tasks(wl.fwd, self.part_cost_for(K), wgw)
+                raise ValueError(f"unknown dense-row mode {mode!r}")
+        if dense is not None:
+            from dataclasses import replace as _dc_replace
+
+            rest = _dc_replace(wl.fwd, blocks=dense[3], brow_ptr=dense[4])
+            tasks = build_tasks(rest, self.part_cost_for(K), wgw)
+        elif chunked is not None:
+            tasks = chunked[0]
+        else:
+            tasks = build_tasks(wl.fwd, self.part_cost_for(K), wgw)
         kt = self.kt_for(K)
         col_sources = self._column_sources()
         has_src = any(len(c) for c in col_sources) and self.enable_injection_jacobian
         h, dldz, hadia, finj, coef, J = self._schedule_arrays(z_grid, col_sources)
         src_slot, src_grid, slots = self._source_tables(col_sources, J)
-        if wide:
-            src_grid = src_grid.transpose(0, 1, 3, 2)  # [slot][J][d][K]
         dcoef, ddiag = self._D_tables
         pscale = pair_species_scale(self.pair_loss_rates_grid, self.spec_man)
 
@@ -698,11 +655,9 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             flags |= _lib.PRH_F_PARTIAL_DIFF
         if has_src:
             flags |= _lib.PRH_F_INJECTION
-        if wide:
-            dense = chunked = None
         blocks = wl.blocks if dense is None else dense[3]
         blocks = blocks if blocks.size else np.zeros((1, 4))
-        sshape = (nspec, d, K) if wide else (nspec, K, d)
+        sshape = (nspec, K, d)
         bufs = {
             "tasks": dv(tasks, np.int32),
             "blocks": dv(blocks, np.int32),
@@ -737,19 +692,13 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             bufs["dense_row"] = dv(row_of, np.int32)
             bufs["fin_tasks"] = dv(fin_tasks, np.int32)
             bufs["dpart"] = torch.empty((ndchunk, len(drows), K, d), dtype=torch.float64, device=dev)
-        if wide:
-            bufs["wtasks"] = dv(wtasks, np.int32)
-            bufs["wheavy"] = dv(wheavy if wheavy.size else np.zeros((1, 3)), np.int32)
-            bufs["wpart"] = torch.empty((max(1, nwslots), d, K), dtype=torch.float64, device=dev)
         if two:
             bufs["mvals_e"] = torch.empty((nwin, stride), dtype=torch.float64, device=dev)
             bufs["phot_e"] = dv(phot_e, np.float64)
             bufs["rpair_e"] = dv(rpair_e, np.float64)
             bufs["escale"] = dv(self._field_split[1], np.float64)
 
-        fwd_wgw = 16 if wide else self.wg_waves_for(K, with_adjoint)
-
-        def make_op(tk, blk, dcf, wide_op=False, wgw=16, dense_op=False):
+        def make_op(tk, blk, dcf, wgw=16, dense_op=False):
             return _lib.PrhOperator(
                 nspec=nspec, d=d, K=K, ntasks=tk.shape[0],
                 tasks=tk.data_ptr() if tk.numel() else None,
@@ -762,11 +711,6 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
                 mvals_e=bufs["mvals_e"].data_ptr() if two else None,
                 rpair_e=bufs["rpair_e"].data_ptr() if two else None,
                 escale=bufs["escale"].data_ptr() if two else None,
-                wide=1 if wide_op else 0,
-                nwtasks=int(wtasks.shape[0]) if wide_op else 0,
-                wtasks=bufs["wtasks"].data_ptr() if wide_op else None,
-                nheavy=int(wheavy.shape[0]) if wide_op else 0, pad2_=0,
-                wheavy=bufs["wheavy"].data_ptr() if wide_op else None,
                 wg_waves=wgw,
                 ndense=(len(dense[0]) if dense is not None else len(chunked[2]) if chunked is not None
                         else 0) if dense_op else 0,
@@ -777,8 +721,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
                 fin_tasks=bufs["fin_tasks"].data_ptr() if (chunked is not None and dense_op) else None,
             )
 
-        op = make_op(bufs["tasks"], bufs["blocks"], bufs["dcoef"], wide_op=wide, wgw=fwd_wgw,
-                     dense_op=True)
+        op = make_op(bufs["tasks"], bufs["blocks"], bufs["dcoef"], wgw=wgw, dense_op=True)
         opT = None
         if with_adjoint:
             tasks_t = build_tasks(wl.tr, self.part_cost_for(K), _lib.WG_WAVES)
@@ -799,7 +742,6 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             x=x.data_ptr(), a=bufs["a"].data_ptr(), fphi=bufs["fphi"].data_ptr(),
             step_ctl=bufs["step_ctl"].data_ptr(), rec_x=None, rec_a=None, rec_fx=None,
             rec_fa=None, rec_first=0, rec_count=0,
-            wpart=bufs["wpart"].data_ptr() if wide else None,
             em=ctypes.addressof(em_struct) if em_struct is not None else None,
             dpart=bufs["dpart"].data_ptr() if "dpart" in bufs else None,
         )
@@ -808,14 +750,12 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             "nsteps": len(h), "layout": wl, "nwin": nwin, "zs": zs, "tasks": tasks, "kt": kt,
             "stream": stream, "stride": stride, "dph": cache.dph, "K": K,
             "fold_rows": stride, "graph": None, "two": two, "z_grid": z_grid,
-            "src_slots": slots, "J": J, "win": win, "wide": wide,
+            "src_slots": slots, "J": J, "win": win,
             "dense_rows": (list(dense[0]) if dense is not None
                            else list(chunked[2]) if chunked is not None else []),
             "dense_mode": "pass" if dense is not None else "chunks" if chunked is not None else None,
             "em": em_struct, "em_bufs": em_bufs,
         }
-        if wide:
-            self._plan.update(wtasks=wtasks, wheavy=wheavy, nwslots=nwslots)
         if getattr(self.backend, "use_hip_graphs", True) and len(h) >= self.graph_chunk:
             self._plan["graph"] = _lib.StepGraph(op, sch, st, self.graph_chunk)
         self.refresh_zs = zs
@@ -942,29 +882,15 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
     def tuning_candidates(self, K):
         """Stage configurations the autotuner times for K right-hand sides:
         right-hand sides per wave (kt), part budget (block cost per wave part,
-        i.e. rows per wave), waves per workgroup, the dense-row mode, and the
-        wide layout when it applies."""
+        i.e. rows per wave), waves per workgroup and the dense-row mode."""
         kts = [kt for kt in (8, 4, 2, 1) if kt <= K][:2]
         out = []
         for kt in kts:
             for pc in (48, 96) if K >= 8 else (12, 24):
                 for wgw in (8, 16):
                     for dense in (("chunks", False) if K >= 8 else (True, "chunks", False)):
-                        out.append({"layout": "narrow", ("kt", K): kt, ("part_cost", K): pc,
+                        out.append({("kt", K): kt, ("part_cost", K): pc,
                                     ("wg_waves", K): wgw, ("dense", K): dense})
-        saved = getattr(self, "_tuned", None)
-        self._tuned = {"layout": "wide"}
-        try:
-            wide_ok = self._use_wide()
-        except ValueError:
-            wide_ok = False
-        finally:
-            if saved is None:
-                del self._tuned
-            else:
-                self._tuned = saved
-        if wide_ok:
-            out.append({"layout": "wide", ("wide_part_cost", K): 96})
         return out
 
     def autotune(self, dz=1e-3, steps=30, candidates=None, repeats=2):

@@ -273,14 +273,13 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
             st.em = ctypes.addressof(em_struct)
         else:
             em_struct = em_bufs = None
-        self._state_wide = False
         self._plan = {
             "op": op, "un": un, "st": st, "x": x, "bufs": bufs, "nsteps": T, "order": order,
             "ref_off": np.asarray(ref_off), "stride": stride, "dph": cache.dph, "K": K,
             "stream": torch.cuda.current_stream(dev).cuda_stream, "layout": wl, "kt": kt,
             "nwin": int(zw.size), "col_steps": nsteps, "graph": None, "em_bufs": em_bufs,
             "live": live, "refold_group": S, "nslot": nslot,
-            "em": em_struct, "two": False, "wide": False,
+            "em": em_struct, "two": False,
         }
         self.timings["setup_host_s"] = time.perf_counter() - t0
         return self._plan

@@ -63,45 +63,42 @@ def test_backend_dispatch_and_gpu_requirement():
             UHECRPropagationSolverETD2(1.0, 0.0, run)
 
 
-def test_abi_rejects_bad_wide_operators_without_touching_the_gpu():
-    """The wide layout's preconditions are checked before any launch."""
+def test_abi_rejects_bad_operators_without_touching_the_gpu():
+    """Operator preconditions are checked before any launch."""
     import ctypes
 
     lib = _lib.load()
     dummy = ctypes.c_int32(0)
     ptr = ctypes.addressof(dummy)
-    op = _lib.PrhOperator(nspec=2, d=64, K=96, ntasks=0, wide=1, nwtasks=1, wtasks=ptr, kt=1)
+    op = _lib.PrhOperator(nspec=2, d=64, K=96, ntasks=0, kt=1)
     sch = _lib.PrhSchedule(nsteps=4)
     st = _lib.PrhState(x=ptr, a=ptr, fphi=ptr, step_ctl=ptr)
     args = (ctypes.byref(op), ctypes.byref(sch), ctypes.byref(st), 0, 1, None)
     assert lib.prh_etd2_steps(*args) != 0
-    assert b"K % 64" in lib.prh_last_error()
-    op.K = 128
-    op.nheavy = 1
-    assert lib.prh_etd2_steps(*args) != 0
-    assert b"partial buffer" in lib.prh_last_error()
-    op.nheavy = 0
-    op.nwtasks = 0
-    assert lib.prh_etd2_steps(*args) != 0
     assert b"empty operator" in lib.prh_last_error()
-    op.nwtasks = 1
+    op.ntasks = 16
+    op.kt = 3
+    assert lib.prh_etd2_steps(*args) != 0
+    assert b"kt must be" in lib.prh_last_error()
+    op.kt = 1
+    op.wg_waves = 4
+    assert lib.prh_etd2_steps(*args) != 0
+    assert b"wg_waves must be" in lib.prh_last_error()
+    op.wg_waves = 8
     adj = _lib.PrhAdjoint(lam=ptr, lam_a=ptr, u=ptr, v=ptr)
     assert lib.prh_adjoint_step(ctypes.byref(op), ctypes.byref(sch), ctypes.byref(adj), 0, None) != 0
-    assert b"narrow layout" in lib.prh_last_error()
+    assert b"16-wave" in lib.prh_last_error()
 
 
 def test_noalloc_entry_point_rejects_before_touching_the_gpu():
-    """Null arguments, the wide layout and a scratch need beyond the static
-    array are refused on the host, before any device call."""
+    """Null arguments and a scratch need beyond the static array are refused on
+    the host, before any device call."""
     import ctypes
 
     lib = _lib.load()
-    op = _lib.PrhOperator(nspec=305, d=64, K=256, wide=0)
+    op = _lib.PrhOperator(nspec=305, d=64, K=256)
     sch = _lib.PrhSchedule(nsteps=1)
     assert lib.prh_etd2_steps_noalloc(None, ctypes.byref(sch), None, 0, 1, None) != 0
     assert b"null" in lib.prh_last_error()
     rc = lib.prh_etd2_steps_noalloc(ctypes.byref(op), ctypes.byref(sch), 8, 0, 1, None)
     assert rc != 0 and b"bytes of scratch" in lib.prh_last_error()
-    op.wide = 1
-    assert lib.prh_etd2_steps_noalloc(ctypes.byref(op), ctypes.byref(sch), 8, 0, 1, None) != 0
-    assert b"narrow" in lib.prh_last_error()

@@ -1,6 +1,6 @@
 """GPU parity of the co-evolved EM cascade: nuclear states and the photon
 spectrum against the CPU oracle (<= 1e-8 relative in every bin above 1e-30 of
-each block's peak), in the narrow (K = 1) and wide (K = 64) layouts."""
+each block's peak), for one column (K = 1) and for K = 64."""
 
 import numpy as np
 import pytest
@@ -47,7 +47,7 @@ def test_em_cascade_proton_single(zi):
     assert e.size == run.em_grid.d and np.array_equal(spec, gam)
 
 
-def test_em_cascade_nuclear_wide_columns():
+def test_em_cascade_nuclear_columns():
     """z 0.1 -> 0.  (After only ~30 steps a few bins hold negative undershoots
     of ~1e-28 of the peak whose value is set by the last bit of exp(h d); a
     1-ulp change of exp on the CPU moves them by up to 4e-6, so short runs are
@@ -56,10 +56,8 @@ def test_em_cascade_nuclear_wide_columns():
     K = 64
     srcs = workloads.batch_sources(run, K, seed=5)
     g = MultiRHSPropagationSolverETD2(0.1, 0.0, run, K=K)
-    g._tuned = {"layout": "wide"}
     for k in range(K):
         g[k].add_source_class(srcs[k])
-    assert g._use_wide()
     g.solve(dz=1e-3)
     o = OracleETD2(0.1, 0.0, run, K=K)
     for k in range(K):
@@ -72,7 +70,7 @@ def test_em_cascade_nuclear_wide_columns():
         dg, nbg = rel_dev(gam[:, k], o.em_state[22][:, k])
         assert nbn > 100 and nbg > 10
         worst_n, worst_g = max(worst_n, dn), max(worst_g, dg)
-    print(f"nuclear+EM wide K=64: nuclear {worst_n:.3e}, photons {worst_g:.3e}")
+    print(f"nuclear+EM K=64: nuclear {worst_n:.3e}, photons {worst_g:.3e}")
     assert worst_n <= TOL and worst_g <= TOL
 
 

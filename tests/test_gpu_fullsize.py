@@ -25,8 +25,8 @@ pytestmark = pytest.mark.gpu
 #: the stage configuration the benchmark's autotuner picks at K = 512 (8
 #: right-hand sides per wave, part budget 96, 8-wave workgroups, dense rows as
 #: ordinary split rows)
-BENCH_PLAN_512 = {"layout": "narrow", ("kt", 512): 8, ("part_cost", 512): 96,
-                  ("wg_waves", 512): 8, ("dense", 512): False}
+BENCH_PLAN_512 = {("kt", 512): 8, ("part_cost", 512): 96, ("wg_waves", 512): 8,
+                  ("dense", 512): False}
 
 
 def test_plain_nuclear_z3_against_fixture():
@@ -56,7 +56,7 @@ def test_batch512_bench_plan_against_fixture_columns():
         s[k].add_source_class(srcs[k])
     s.solve(dz=1e-3)
     plan = s._plan
-    assert plan["kt"] == 8 and plan["dense_mode"] is None and not plan["wide"]
+    assert plan["kt"] == 8 and plan["dense_mode"] is None
     assert plan["op"].wg_waves == 8 and plan["nwin"] == 288
     worst = 0.0
     for i, k in enumerate(g["columns"]):
@@ -82,7 +82,7 @@ def test_em_cascade_nuclear_narrow_k16():
     for k in range(K):
         g[k].add_source_class(srcs[k])
     g.solve(dz=1e-3)
-    assert not g._plan["wide"] and g._plan["em"] is not None
+    assert g._plan["em"] is not None
     o = OracleETD2(0.1, 0.0, run, K=K)
     for k in range(K):
         o.add_source_class(srcs[k], k)

@@ -136,50 +136,6 @@ def _multi(run, zi, K, srcs, tuned=None):
     return g
 
 
-def test_wide_layout_matches_oracle_columns():
-    """Lanes = right-hand sides (K = 64): every column against the oracle."""
-    run = workloads.nuclear_run()
-    K = 64
-    srcs = workloads.batch_sources(run, K, seed=3)
-    g = _multi(run, 0.1, K, srcs, {"layout": "wide"})
-    assert g._use_wide()
-    r = OracleETD2(0.1, 0.0, run, K=K)
-    for k in range(K):
-        r.add_source_class(srcs[k], k)
-    g.solve(dz=1e-3)
-    assert g._state_wide
-    r.solve(dz=1e-3)
-    worst = 0.0
-    for k in range(K):
-        dev, nb = rel_dev(g[k].state, r.state[:, k])
-        assert nb > 100
-        worst = max(worst, dev)
-    print(f"wide K=64: max rel dev over columns {worst:.3e}")
-    assert worst <= TOL
-
-
-@pytest.mark.parametrize("part_cost", [8, 24, 1000])
-def test_wide_layout_matches_narrow(part_cost):
-    """K = 128: wide (several part budgets, graph and direct launches) against
-    the narrow layout; heavy-row splitting changes only the summation order."""
-    run = workloads.nuclear_run()
-    K = 128
-    srcs = workloads.batch_sources(run, K, seed=4)
-    narrow = _multi(run, 0.15, K, srcs, {"layout": "narrow"})
-    narrow.solve(dz=1e-3)
-    states = []
-    for graphs in (True, False):
-        run.backend.use_hip_graphs = graphs
-        g = _multi(run, 0.15, K, srcs, {"layout": "wide", ("wide_part_cost", K): part_cost})
-        g.solve(dz=1e-3)
-        states.append(g.state.copy())
-    run.backend.use_hip_graphs = True
-    assert np.array_equal(states[0], states[1])
-    for k in range(K):
-        dev, nb = rel_dev(states[0][:, k], narrow.state[:, k])
-        assert nb > 100 and dev <= TOL, (k, dev)
-
-
 @pytest.mark.parametrize("mode", ["pass", "chunks"])
 def test_dense_rows_pass_matches_oracle_and_plain_tasks(mode):
     """K = 64 narrow with the dense rows (protons, He-4) applied by the

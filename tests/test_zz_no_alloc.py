@@ -54,7 +54,7 @@ def test_noalloc_matches_existing_entry_point_twice():
         assert np.array_equal(ref, got)
 
 
-def test_noalloc_rejects_oversized_and_wide():
+def test_noalloc_rejects_oversized():
     run = workloads.nuclear_run()
     s, plan = _plan(run, 256, 0.02, seed=33)  # 2 x 40 MB stage vectors > 64 MiB
     lib = _lib.load()
@@ -63,8 +63,3 @@ def test_noalloc_rejects_oversized_and_wide():
     assert rc != 0
     msg = lib.prh_last_error().decode()
     assert "bytes of scratch" in msg and str(_lib.SCRATCH_BYTES) in msg
-    op = plan["op"]
-    op.wide = 1
-    rc = lib.prh_etd2_steps_noalloc(ctypes.byref(op), ctypes.byref(plan["sch"]),
-                                    plan["x"].data_ptr(), 0, 1, plan["stream"])
-    assert rc != 0 and "narrow" in lib.prh_last_error().decode()
