@@ -137,14 +137,21 @@ typedef struct prh_em {
   const double* rbh;    /* [nwin][d_em][d] Bethe-Heitler e+- shape (one sign) */
   const double* tg;     /* [nwin][d_em][d_em] photon transfer T_gamma */
   const double* te;     /* [nwin][d_em][d_em] e+- transfer T_electron */
-  double* gam;          /* [d_em][K] photon state, updated in place every step */
+  /* [d_em][K] photon state.  Shared-schedule steps alternate: step t reads
+   * gam and writes gam2 for even t, the reverse for odd t (after n steps the
+   * photons are in gam2 when n is odd); the uneven batch updates gam in place. */
+  double* gam;
   double* wbh;          /* [d][K] scratch: Bethe-Heitler weights of the step */
-  double* gam2;         /* [d_em][K] scratch: photons after the ETD2 step, before the transfer */
+  double* gam2;         /* [d_em][K] the other photon buffer (shared schedule) */
   /* [nwin][d_em][d_em + d] per window [T_gamma | 2 T_electron R_BH]
-   * (prh_em_transfer_stack): the step's transfer is one product with
-   * [gam2; wbh].  Shared-schedule steps need it; the uneven batch uses tg, te
-   * and rbh directly. */
+   * (prh_em_transfer_stack); shared-schedule steps read its Bethe-Heitler
+   * columns, the uneven batch uses tg, te and rbh directly. */
   const double* tstack;
+  /* [nsteps][d_em][d_em] per step T_gamma[win(t)] P(t), P(t) the photon ETD2
+   * step as a matrix (prh_em_step_transfer): the shared-schedule EM step is
+   * one product [qstep[t] | 2 T_electron R_BH] [photons; Bethe-Heitler
+   * weights].  Not used by the uneven batch. */
+  const double* qstep;
 } prh_em;
 
 /* Per window w: out[w] = [T_gamma[w] | 2 T_electron[w] R_BH[w]] ([d_em][d_em + d]),
@@ -154,6 +161,15 @@ typedef struct prh_em {
  * _apply_em_cascade, propagation.py:555-571, 620-665, as one product). */
 int32_t prh_em_transfer_stack(const double* tg, const double* te, const double* rbh,
                               int32_t nwin, int32_t d_em, int32_t d, double* out, void* stream);
+
+/* qstep[t] = T_gamma[win(t)] P(t) for every step t of the schedule
+ * ([nsteps][d_em][d_em], caller-allocated): P(t) is the photon ETD2 step of
+ * the reference's EM block (its two stages with the adiabatic FD operator,
+ * propagation.py:867-878 with etd2.py:412-439) written as the matrix it is,
+ * so that T_gamma applied after the step (_apply_em_cascade,
+ * propagation.py:555-571) and the step itself become one product. */
+int32_t prh_em_step_transfer(const prh_operator* op, const prh_schedule* sch, const prh_em* em,
+                             double* qstep, void* stream);
 
 /* Opaque handle of a captured step graph (prh_graph_create, prh_uneven_graph_create). */
 typedef struct prh_graph prh_graph;

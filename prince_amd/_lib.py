@@ -35,6 +35,7 @@ EXPORTED = (
     "prh_cascade_bh",
     "prh_toeplitz_rows",
     "prh_em_transfer_stack",
+    "prh_em_step_transfer",
 )
 
 WG_WAVES = 16
@@ -134,6 +135,7 @@ class PrhEm(ctypes.Structure):
         ("wbh", _p),
         ("gam2", _p),
         ("tstack", _p),
+        ("qstep", _p),
     ]
 
 
@@ -280,6 +282,9 @@ def load():
     lib.prh_toeplitz_rows.argtypes = [ctypes.POINTER(PrhToeplitz), _p, _i64, _p, _p]
     lib.prh_em_transfer_stack.restype = _i32
     lib.prh_em_transfer_stack.argtypes = [_p, _p, _p, _i32, _i32, _i32, _p, _p]
+    lib.prh_em_step_transfer.restype = _i32
+    lib.prh_em_step_transfer.argtypes = [
+        ctypes.POINTER(PrhOperator), ctypes.POINTER(PrhSchedule), ctypes.POINTER(PrhEm), _p, _p]
     lib.prh_cascade_bh.restype = _i32
     lib.prh_cascade_bh.argtypes = [ctypes.POINTER(PrhCascade), _p, _i32, _p, _p]
     lib.prh_etd2_stage.restype = _i32

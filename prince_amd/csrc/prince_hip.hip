@@ -1082,8 +1082,9 @@ __device__ __forceinline__ void resolve_step(const prh_state& st, int step_local
 }
 
 // ---------------------------------------------------------------------------
-// EM sector (photons, e+-): three launches per step after stage 2 -- the
-// Bethe-Heitler pair weights, the photon ETD2 step and the cascade transfer.
+// EM sector (photons, e+-): two launches per step after stage 2 -- the
+// Bethe-Heitler pair weights, then the photon ETD2 step and the cascade
+// transfer as one product with the per-step matrix T_gamma P(t).
 // ---------------------------------------------------------------------------
 
 // Bethe-Heitler weights w[j][k] = bhw[win][j] * sum_s cbh[s] x[s][k][j]:
@@ -1125,119 +1126,135 @@ __global__ void __launch_bounds__(256) em_bh_weights_bins(prh_operator op, prh_s
   em.wbh[(int64_t)lane * K + k] = em.bhw[(int64_t)w * d + lane] * tot;
 }
 
-// Photon ETD2 step of every column (gam -> gam2): one workgroup per column,
-// one thread per EM bin, both stages through LDS (K workgroups: the grid
-// fills the chip).  The stencil, phi factors and update in the order of the
-// nuclear stage epilogue.
-__global__ void __launch_bounds__(256) em_photon_kernel(prh_operator op, prh_schedule sch,
-                                                       prh_state st, prh_em em, int step_local,
-                                                       int use_ctl) {
-  __shared__ double g0[PRH_EM_MAX];
-  __shared__ double ga[PRH_EM_MAX];
-  int step;
-  bool live;
-  resolve_step(st, step_local, use_ctl, step, live);
-  if (!live) return;
-  const int K = op.K;
+// Per-step photon transfer of the shared-schedule EM step:
+//   qstep[t] = T_gamma[win(t)] P(t),
+// P(t) the photon ETD2 step of step t as a matrix.  The photon stages are
+// linear in the photons g:  a = e g + h phi1 F g,  g2 = a + h phi2 (F a - F g)
+// with F the FD stencil on kappa g (F[i][j] = dl dcoef[i][j - i] kappa_j,
+// kappa = H cm2sec E, 0 < |j - i| <= 3) and e, phi1, phi2 of h d, d = dl kappa
+// ddiag, so  P = A + h diag(phi2) F (A - I),  A = diag(e) + h diag(phi1) F,
+// a band of 13 diagonals.  One workgroup per step: the band of P in LDS, then
+// every entry of T_gamma P (13 terms each).
+__global__ void __launch_bounds__(256) em_qstep_kernel(prh_operator op, prh_schedule sch,
+                                                      prh_em em, double* __restrict__ qstep) {
+  __shared__ double pb[PRH_EM_MAX][13];
+  __shared__ double se[PRH_EM_MAX], sp1[PRH_EM_MAX], sp2[PRH_EM_MAX], skap[PRH_EM_MAX];
+  const int t = blockIdx.x;
   const int de = em.d_em;
-  const int k = blockIdx.x;
-  const int i = threadIdx.x;
-  const double hh = sch.h[step];
-  const double dl = sch.dldz[step];
-  const double had = sch.hadia[step];
+  const double hh = sch.h[t];
+  const double dl = sch.dldz[t];
+  const double had = sch.hadia[t];
   const bool pdiff = (op.flags & PRH_F_PARTIAL_DIFF) != 0;
-  if (i < de) g0[i] = em.gam[(int64_t)i * K + k];
-  __syncthreads();
-  // F(y)_i = dl * sum_{o != 0} D[i][i+o] kappa_{i+o} y_{i+o},  kappa = H cm2sec E
-  auto stencil = [&](const double* y) {
-    double f = 0.0;
-    if (pdiff) {
-#pragma unroll
-      for (int o = -3; o <= 3; ++o) {
-        const int j = i + o;
-        if (o == 0 || j < 0 || j >= de) continue;
-        const double cf = em.dcoef[i * 7 + o + 3];
-        if (cf != 0.0) f += cf * ((had * em.egrid[j]) * y[j]);
-      }
-    }
-    return f * dl;
-  };
-  double f0 = 0.0, p2 = 0.0;
-  if (i < de) {
+  for (int i = threadIdx.x; i < de; i += blockDim.x) {
+    const double kap = had * em.egrid[i];
     double dv = 0.0;
     if (pdiff) {
-      double kx = (had * em.egrid[i]) * em.ddiag[i];
+      double kx = kap * em.ddiag[i];
       dv = kx * dl;
     }
     const double hd = dv * hh;
     const double e = exp(hd);
-    const double p1 = phi1_of(hd, e);
-    p2 = phi2_of(hd, e);
-    f0 = stencil(g0);
-    double a = e * g0[i];
-    double sc = p1 * f0;
-    sc = sc * hh;
-    ga[i] = a + sc;
+    se[i] = e;
+    sp1[i] = phi1_of(hd, e);
+    sp2[i] = phi2_of(hd, e);
+    skap[i] = kap;
   }
   __syncthreads();
-  if (i < de) {
-    const double fa = stencil(ga);
-    double sc = fa - f0;
-    sc = sc * hh;
-    sc = sc * p2;
-    em.gam2[(int64_t)i * K + k] = ga[i] + sc;
+  // F[a][b] (zero off the stencil and without the partial-differential term)
+  auto F = [&](int a, int b) -> double {
+    const int o = b - a;
+    if (!pdiff || o == 0 || o < -3 || o > 3 || b < 0 || b >= de) return 0.0;
+    return dl * (em.dcoef[a * 7 + o + 3] * skap[b]);
+  };
+  for (int r = threadIdx.x; r < de * 13; r += blockDim.x) {
+    const int m = r / 13, o = r % 13 - 6, j = m + o;
+    double v = 0.0;
+    if (j >= 0 && j < de) {
+      v = (o == 0 ? se[m] : 0.0) + hh * sp1[m] * F(m, j);
+      double s = 0.0;
+      for (int on = -3; on <= 3; ++on) {
+        const int n = m + on;
+        if (on == 0 || n < 0 || n >= de) continue;
+        const int oj = j - n;
+        if (oj < -3 || oj > 3) continue;
+        const double amd = oj == 0 ? se[n] - 1.0 : hh * sp1[n] * F(n, j);
+        s += F(m, n) * amd;
+      }
+      v += hh * sp2[m] * s;
+    }
+    pb[m][o + 6] = v;
+  }
+  __syncthreads();
+  const int w = sch.win[t];
+  const double* T = em.tg + (int64_t)w * de * de;
+  double* Q = qstep + (int64_t)t * de * de;
+  for (int r = threadIdx.x; r < de * de; r += blockDim.x) {
+    const int i = r / de, j = r % de;
+    const int m0 = j - 6 > 0 ? j - 6 : 0, m1 = j + 6 < de - 1 ? j + 6 : de - 1;
+    double q = 0.0;
+    for (int m = m0; m <= m1; ++m) q += T[(int64_t)i * de + m] * pb[m][j - m + 6];
+    Q[r] = q;
   }
 }
 
-// Transfer of the step as one FP64 MFMA product per 16 x 16 tile:
-//   gam[i][k] = sum_c S[w][i][c] y[c][k],  S = [T_gamma | 2 T_electron R_BH],
-//   y = [gam2 (photons after the ETD2 step); wbh (Bethe-Heitler weights)]
-// one wave per (row tile, column tile), contraction length d_em + d.
-__global__ void __launch_bounds__(256) em_transfer_mfma(prh_operator op, prh_schedule sch,
-                                                       prh_state st, prh_em em, int step_local,
-                                                       int use_ctl) {
+// The EM step's transfer as one FP64 MFMA product per 16 x 16 tile:
+//   out[i][k] = sum_c S[i][c] y[c][k],  S = [qstep[t] | 2 T_electron R_BH],
+//   y = [photons before the step; Bethe-Heitler weights wbh],
+// contraction length d_em + d split over the 4 waves of the tile's
+// workgroup (summed through LDS in wave order).  The photon state alternates:
+// step t reads gam and writes gam2 for even t, the reverse for odd t.
+__global__ void __launch_bounds__(256) em_step_mfma(prh_operator op, prh_schedule sch,
+                                                   prh_state st, prh_em em, int step_local,
+                                                   int use_ctl) {
+  __shared__ double red[4][4][WAVE];
   int step;
   bool live;
   resolve_step(st, step_local, use_ctl, step, live);
   if (!live) return;
   const int K = op.K;
   const int de = em.d_em;
-  const int kd = de + op.d;
+  const int d = op.d;
+  const int kd = de + d;
   const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t wave = (int64_t)blockIdx.x * 4 + wave_id();
+  const int wv = wave_id();
   const int nct = (K + 15) / 16;
-  const int nrt = (de + 15) / 16;
-  if (wave >= (int64_t)nrt * nct) return;
-  const int rt = (int)(wave / nct), ct = (int)(wave % nct);
+  const int rt = blockIdx.x / nct, ct = blockIdx.x % nct;
   const int col = lane & 15, kq = lane >> 4;
   const int i = rt * 16 + col;
+  const int ic = i < de ? i : de - 1;
   const int kc = ct * 16 + col;
-  const int w = sch.win[step];
-  const double* S = em.tstack + (int64_t)w * de * kd + (int64_t)(i < de ? i : de - 1) * kd;
   const int kcc = kc < K ? kc : K - 1;
-  double4_t acc = {0.0, 0.0, 0.0, 0.0};
-  // the loads of 10 k-steps are issued before their products
-  constexpr int UK = 10;
-  for (int c0 = 0; c0 < kd; c0 += 4 * UK) {
-    double a[UK], b[UK];
+  const int w = sch.win[step];
+  const double* Qr = em.qstep + ((int64_t)step * de + ic) * de;
+  const double* Sr = em.tstack + ((int64_t)w * de + ic) * kd;
+  const double* gin = (step & 1) ? em.gam2 : em.gam;
+  double* gout = (step & 1) ? em.gam : em.gam2;
+  // this wave's k-steps (4 contraction indices each), loads first
+  constexpr int KS = 13;
+  const int nks = (kd + 3) / 4;
+  const int s0 = wv * nks / 4, s1 = (wv + 1) * nks / 4;
+  double a[KS], b[KS];
 #pragma unroll
-    for (int u = 0; u < UK; ++u) {
-      const int cc = c0 + 4 * u + kq;
-      a[u] = (i < de && cc < kd) ? S[cc] : 0.0;
-      b[u] = 0.0;
-      if (cc < de)
-        b[u] = em.gam2[(int64_t)cc * K + kcc];
-      else if (cc < kd)
-        b[u] = em.wbh[(int64_t)(cc - de) * K + kcc];
-    }
-#pragma unroll
-    for (int u = 0; u < UK; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+  for (int u = 0; u < KS; ++u) {
+    const int c = 4 * (s0 + u) + kq;
+    a[u] = 0.0;
+    b[u] = 0.0;
+    if (s0 + u < s1 && i < de && c < kd) a[u] = c < de ? Qr[c] : Sr[c];
+    if (s0 + u < s1 && c < kd) b[u] = c < de ? gin[(int64_t)c * K + kcc] : em.wbh[(int64_t)(c - de) * K + kcc];
   }
-  if (kc >= K) return;
+  double4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < KS; ++u)
+    if (s0 + u < s1) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) red[wv][v][lane] = acc[v];
+  __syncthreads();
+  if (wv != 0 || kc >= K) return;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int row = rt * 16 + kq + 4 * v;
-    if (row < de) em.gam[(int64_t)row * K + kc] = acc[v];
+    const double sum = ((red[0][v][lane] + red[1][v][lane]) + red[2][v][lane]) + red[3][v][lane];
+    if (row < de) gout[(int64_t)row * K + kc] = sum;
   }
 }
 
@@ -1269,11 +1286,9 @@ void launch_em(const prh_operator& op, const prh_schedule& sch, const prh_state&
   const prh_em& em = *st.em;
   hipLaunchKernelGGL(em_bh_weights_bins, dim3((unsigned)op.K), dim3(256), 0, s, op, sch, st, em,
                      step_local, use_ctl);
-  hipLaunchKernelGGL(em_photon_kernel, dim3((unsigned)op.K), dim3(192), 0, s, op, sch, st, em,
-                     step_local, use_ctl);
   const int64_t tiles = (int64_t)((em.d_em + 15) / 16) * ((op.K + 15) / 16);
-  hipLaunchKernelGGL(em_transfer_mfma, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, op, sch,
-                     st, em, step_local, use_ctl);
+  hipLaunchKernelGGL(em_step_mfma, dim3((unsigned)tiles), dim3(256), 0, s, op, sch, st, em,
+                     step_local, use_ctl);
 }
 
 int32_t check_uneven(const prh_operator* op, const prh_uneven* un, const prh_state* st,
@@ -1303,7 +1318,7 @@ int32_t validate_em(const prh_operator* op, const prh_state* st, const char* who
   if (em->d_em < 1 || em->d_em > PRH_EM_MAX)
     return set_error(std::string(who) + ": d_em must be in [1, 144]");
   if (!em->egrid || !em->dcoef || !em->ddiag || !em->cbh || !em->bhw || !em->rbh || !em->tg ||
-      !em->te || !em->gam || !em->gam2 || !em->wbh || !em->tstack)
+      !em->te || !em->gam || !em->gam2 || !em->wbh || !em->tstack || !em->qstep)
     return set_error(std::string(who) + ": EM sector with a null buffer");
   if (em->nwin < 1) return set_error(std::string(who) + ": EM sector without windows");
   (void)op;
@@ -2698,6 +2713,21 @@ int32_t prh_em_transfer_stack(const double* tg, const double* te, const double* 
   hipLaunchKernelGGL(em_stack_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, tg, te, rbh, (int)nwin, (int)d_em, (int)d, out);
   return check_hip(hipGetLastError(), "prh_em_transfer_stack launch");
+}
+
+int32_t prh_em_step_transfer(const prh_operator* op, const prh_schedule* sch, const prh_em* em,
+                             double* qstep, void* stream) {
+  if (!op || !sch || !em || !qstep) return set_error("prh_em_step_transfer: null argument");
+  if (em->d_em < 1 || em->d_em > PRH_EM_MAX)
+    return set_error("prh_em_step_transfer: d_em must be in [1, 144]");
+  if (!em->egrid || !em->dcoef || !em->ddiag || !em->tg)
+    return set_error("prh_em_step_transfer: EM sector with a null buffer");
+  if (!sch->h || !sch->dldz || !sch->hadia || !sch->win || sch->nsteps < 0)
+    return set_error("prh_em_step_transfer: null schedule buffer");
+  if (sch->nsteps == 0) return 0;
+  hipLaunchKernelGGL(em_qstep_kernel, dim3((unsigned)sch->nsteps), dim3(256), 0,
+                     (hipStream_t)stream, *op, *sch, *em, qstep);
+  return check_hip(hipGetLastError(), "prh_em_step_transfer launch");
 }
 
 int32_t prh_toeplitz_rows(const prh_toeplitz* tz, const int64_t* sel, int64_t nsel, double* out,

@@ -398,9 +398,12 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         return np.ascontiguousarray(host[:, 0] if K == 1 else host)
 
     def em_host_state(self, plan=None):
-        """EM blocks (e+, e-, photon) of the last solve as a host array [3 d_em, K]."""
+        """EM blocks (e+, e-, photon) of the last solve as a host array [3 d_em, K]
+        (the photons alternate between two buffers: after an odd number of
+        steps they are in ``gam2``)."""
         plan = plan if plan is not None else self._plan
-        gam = plan["em_bufs"]["gam"].cpu().numpy()
+        eb = plan["em_bufs"]
+        gam = (eb["gam2"] if plan.get("steps_run", plan["nsteps"]) % 2 else eb["gam"]).cpu().numpy()
         z = np.zeros_like(gam)
         return np.concatenate([z, z, gam], axis=0)
 
@@ -738,6 +741,12 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         em_struct, em_bufs = (None, None)
         if self.enable_em_cascade:
             em_struct, em_bufs = self._em_setup(zs, K, dv)
+            de = self.prince_run.em_grid.d
+            em_bufs["qstep"] = torch.empty((len(h), de, de), dtype=torch.float64, device=dev)
+            em_struct.qstep = em_bufs["qstep"].data_ptr()
+            _lib.check(self._lib.prh_em_step_transfer(
+                ctypes.byref(op), ctypes.byref(sch), ctypes.byref(em_struct),
+                em_bufs["qstep"].data_ptr(), stream), "EM step transfers")
         st = _lib.PrhState(
             x=x.data_ptr(), a=bufs["a"].data_ptr(), fphi=bufs["fphi"].data_ptr(),
             step_ctl=bufs["step_ctl"].data_ptr(), rec_x=None, rec_a=None, rec_fx=None,
@@ -934,6 +943,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         return table
 
     def _launch_steps(self, plan, k_begin, k_end, use_graph=True):
+        plan["steps_run"] = k_end  # the EM photons' buffer after the last step
         if use_graph and plan["graph"] is not None:
             plan["graph"].run(k_begin, k_end, plan["stream"])
             return
@@ -1059,6 +1069,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         torch = _torch()
         plan = self._ensure_apply_F(z_grid)
         plan["x"].copy_(state)
+        self._reset_em_state(plan)
         self.fold_windows(plan)
         self._launch_steps(plan, 0, plan["nsteps"])
         state.copy_(plan["x"])
