@@ -60,6 +60,9 @@ def parse():
     p.add_argument("--em-batch", type=int, default=512)
     p.add_argument("--no-autotune", action="store_true")
     p.add_argument("--uneven-batch", type=int, default=1024)
+    p.add_argument("--uneven-steps", type=int, default=None,
+                   help="march only the first N global steps of the uneven legs (profiler runs; "
+                        "no column check)")
     p.add_argument("--z-max", type=float, default=3.0,
                    help="initial redshift (3.0 is the benchmark; smaller values shorten profiler runs)")
     p.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "etd2_stage_K512_traffic.json"),
@@ -259,7 +262,7 @@ def adjoint_leg(K, torch, log):
     }
 
 
-def uneven_leg(run, K, torch, log, seed=0):
+def uneven_leg(run, K, torch, log, seed=0, steps=None):
     """Uneven batch (K columns, z_max ~ U[0.1, 5], EBL scale ~ U[0.6, 1.4], the
     batched source draws): one timed device solve after a 128-step warm-up, and
     two columns checked against a plain single-column solve of the same field."""
@@ -280,7 +283,7 @@ def uneven_leg(run, K, torch, log, seed=0):
     g.execute(plan, 0, min(128, plan["nsteps"]))  # warm-up; captures the chunk graphs
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    g.execute(plan)
+    g.execute(plan, 0, steps)
     torch.cuda.synchronize()
     t_run = time.perf_counter() - t0
     nspec, d = run.spec_man.nspec, run.cr_grid.d
@@ -288,7 +291,7 @@ def uneven_leg(run, K, torch, log, seed=0):
     state = np.empty_like(host)
     state[:, plan["order"]] = host
     devs = []
-    for k in (int(np.argmax(z_max)), int(np.argmin(z_max))):
+    for k in (int(np.argmax(z_max)), int(np.argmin(z_max))) if steps is None else ():
         one = MultiRHSPropagationSolverETD2(float(z_max[k]), 0.0, run, K=1)
         one.set_field_scales(1, [scale[k]])
         one[0].add_source_class(srcs[k])
@@ -296,20 +299,25 @@ def uneven_leg(run, K, torch, log, seed=0):
         ref = np.asarray(one.state).reshape(-1)
         m = np.abs(ref) > 1e-30 * np.abs(ref).max()
         devs.append(float(np.max(np.abs(state[m, k] - ref[m]) / np.abs(ref[m]))))
-    steps = int(plan["col_steps"].sum())
-    log(f"uneven K={K}: {plan['nsteps']} steps (column total {steps}), {plan['nwin']} windows, "
-        f"setup {t_setup:.1f}s, device {t_run*1e3:.0f} ms -> {K/t_run:.1f} solves/s; "
-        f"column check {max(devs):.2e}")
+    csteps = int(plan["col_steps"].sum())
+    check = max(devs) if devs else None
+    log(f"uneven K={K}: {plan['nsteps']} steps (column total {csteps}), {plan['nwin']} windows, "
+        f"setup {t_setup:.1f}s, device {t_run*1e3:.0f} ms"
+        + (f" -> {K/t_run:.1f} solves/s; column check {check:.2e}" if steps is None
+           else f" for the first {steps} steps"))
     return {
         "K": K, "z_max_range": [0.1, 5.0], "ebl_scale_range": [0.6, 1.4],
-        "global_steps": plan["nsteps"], "column_steps_total": steps, "windows": plan["nwin"],
+        "global_steps": plan["nsteps"], "column_steps_total": csteps, "windows": plan["nwin"],
+        "refold_group_steps": plan["refold_group"],
+        "steps_timed": steps if steps is not None else plan["nsteps"],
         "setup_host_s": round(t_setup, 2), "ms_per_batch": round(t_run * 1e3, 1),
-        "solves_per_s": round(K / t_run, 2),
-        "column_max_rel_dev_vs_two_field_solve": max(devs),
+        "solves_per_s": round(K / t_run, 2) if steps is None else None,
+        "column_steps_per_s": round(csteps / t_run, 1) if steps is None else None,
+        "column_max_rel_dev_vs_two_field_solve": check,
     }
 
 
-def uneven_em_leg(K, torch, log, seed=0):
+def uneven_em_leg(K, torch, log, seed=0, steps=None):
     """The uneven batch with the co-evolved EM cascade (K columns, z_max ~
     U[0.1, 5], EBL scale ~ U[0.6, 1.4]): the cascade transfer and Bethe-Heitler
     shapes of every column window are built on the device before the march.
@@ -334,7 +342,7 @@ def uneven_em_leg(K, torch, log, seed=0):
     g.execute(plan, 0, min(128, plan["nsteps"]))  # warm-up; captures the chunk graphs
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    g.execute(plan)
+    g.execute(plan, 0, steps)
     torch.cuda.synchronize()
     t_run = time.perf_counter() - t0
     iters = plan["em_bufs"]["iters"].cpu().numpy()
@@ -344,7 +352,7 @@ def uneven_em_leg(K, torch, log, seed=0):
     state = np.empty((nspec * d + de, K))
     state[:, plan["order"]] = np.concatenate([host, gam], axis=0)
     devs = []
-    for k in (int(np.argmin(z_max)), int(np.argsort(z_max)[K // 2])):
+    for k in (int(np.argmin(z_max)), int(np.argsort(z_max)[K // 2])) if steps is None else ():
         one = UnevenBatchSolverETD2(z_max[k:k + 1], run, ebl_scale=scale[k:k + 1])
         one[0].add_source_class(srcs[k])
         one.solve(1e-3)
@@ -353,22 +361,26 @@ def uneven_em_leg(K, torch, log, seed=0):
             r, t = ref[part], state[part, k]
             m = np.abs(r) > 1e-30 * np.abs(r).max()
             devs.append(float(np.max(np.abs(t[m] - r[m]) / np.abs(r[m]))))
-    steps = int(plan["col_steps"].sum())
+    csteps = int(plan["col_steps"].sum())
     del g, plan
     torch.cuda.empty_cache()
-    log(f"uneven+EM K={K}: {steps} column steps, {len(iters)} windows; setup {t_setup:.1f}s of which "
-        f"device window operators {t_build:.1f}s; device march {t_run*1e3:.0f} ms -> "
-        f"{K/t_run:.1f} solves/s ({K/(t_run + t_build):.1f} with the operator builds); "
-        f"column check {max(devs):.2e}")
+    check = max(devs) if devs else None
+    log(f"uneven+EM K={K}: {csteps} column steps, {len(iters)} windows; setup {t_setup:.1f}s of which "
+        f"device window operators {t_build:.1f}s; device march {t_run*1e3:.0f} ms"
+        + (f" -> {K/t_run:.1f} solves/s ({K/(t_run + t_build):.1f} with the operator builds); "
+           f"column check {check:.2e}" if steps is None else f" for the first {steps} steps"))
+    if steps is not None:
+        return {"K": K, "steps_timed": steps, "ms": round(t_run * 1e3, 1),
+                "em_operator_build_s": round(t_build, 2)}
     return {
         "K": K, "z_max_range": [0.1, 5.0], "ebl_scale_range": [0.6, 1.4],
-        "windows": int(len(iters)), "column_steps_total": steps,
+        "windows": int(len(iters)), "column_steps_total": csteps,
         "setup_s": round(t_setup, 2), "em_operator_build_s": round(t_build, 2),
         "em_operator_builds_per_s": round(len(iters) / t_build, 1) if t_build > 0 else None,
         "cascade_iterations_mean": float(np.mean(iters)), "cascade_iterations_max": int(iters.max()),
         "ms_per_batch": round(t_run * 1e3, 1), "solves_per_s": round(K / t_run, 2),
         "solves_per_s_with_operator_builds": round(K / (t_run + t_build), 2),
-        "column_max_rel_dev_vs_one_column_batch": max(devs),
+        "column_max_rel_dev_vs_one_column_batch": check,
     }
 
 
@@ -634,7 +646,7 @@ def main():
     # ---------------- uneven batch (per-column z_max and EBL scale)
     uneven = None
     if not args.no_uneven:
-        uneven = uneven_leg(run, args.uneven_batch, torch, log)
+        uneven = uneven_leg(run, args.uneven_batch, torch, log, steps=args.uneven_steps)
 
     # ---------------- EM cascade (shared schedule)
     em = None
@@ -644,7 +656,7 @@ def main():
     # ---------------- uneven batch with the EM cascade (C3 as specified)
     uneven_em = None
     if not args.no_uneven_em:
-        uneven_em = uneven_em_leg(args.uneven_em_batch, torch, log)
+        uneven_em = uneven_em_leg(args.uneven_em_batch, torch, log, steps=args.uneven_steps)
 
     # ---------------- CPU baseline: the FP64 oracle on the host cores
     cpu = None

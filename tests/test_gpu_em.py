@@ -47,7 +47,8 @@ def test_em_cascade_proton_single(zi):
     assert e.size == run.em_grid.d and np.array_equal(spec, gam)
 
 
-def test_em_cascade_nuclear_columns():
+@pytest.mark.parametrize("dense", ["chunks", False])
+def test_em_cascade_nuclear_columns(dense):
     """z 0.1 -> 0.  (After only ~30 steps a few bins hold negative undershoots
     of ~1e-28 of the peak whose value is set by the last bit of exp(h d); a
     1-ulp change of exp on the CPU moves them by up to 4e-6, so short runs are
@@ -56,9 +57,11 @@ def test_em_cascade_nuclear_columns():
     K = 64
     srcs = workloads.batch_sources(run, K, seed=5)
     g = MultiRHSPropagationSolverETD2(0.1, 0.0, run, K=K)
+    g._tuned = {("dense", K): dense}
     for k in range(K):
         g[k].add_source_class(srcs[k])
     g.solve(dz=1e-3)
+    assert g._plan["dense_mode"] == (dense or None)
     o = OracleETD2(0.1, 0.0, run, K=K)
     for k in range(K):
         o.add_source_class(srcs[k], k)
@@ -70,7 +73,7 @@ def test_em_cascade_nuclear_columns():
         dg, nbg = rel_dev(gam[:, k], o.em_state[22][:, k])
         assert nbn > 100 and nbg > 10
         worst_n, worst_g = max(worst_n, dn), max(worst_g, dg)
-    print(f"nuclear+EM K=64: nuclear {worst_n:.3e}, photons {worst_g:.3e}")
+    print(f"nuclear+EM K=64 dense={dense}: nuclear {worst_n:.3e}, photons {worst_g:.3e}")
     assert worst_n <= TOL and worst_g <= TOL
 
 
