@@ -319,8 +319,9 @@ int32_t prh_adjoint_steps(const prh_operator* opT, const prh_schedule* sch, cons
                           int32_t k_begin, int32_t k_end, void* stream);
 /* One EM-sector step (photon ETD2 step, Bethe-Heitler injection, cascade
  * transfer) after the nuclear step `step`; `st` supplies the post-step nuclear
- * state.  prh_etd2_steps, prh_etd2_stage (stage 2) and graphs run it by
- * themselves when st->em is set.  Replaces the step hook of the reference's
+ * state, the photons move between em->gam and em->gam2 by the parity of
+ * `step` (see prh_em).  prh_etd2_steps, prh_etd2_stage (stage 2) and graphs
+ * run it by themselves when st->em is set.  Replaces the step hook of the reference's
  * solve loop (propagation.py:867-878): _inject_bh_pairs (620-665) followed by
  * _apply_em_cascade (555-571), with the photon blocks of the ETD2 step
  * (etd2.py:412-439). */
