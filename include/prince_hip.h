@@ -437,7 +437,7 @@ typedef struct prh_cascade {
  * contracted with the field-free tensors (prh_fold_gemm): IC emission
  * [n][n], pair injection [n][n], IC power [n], IC rate [n], sum w n,
  * sum w eps n; dtau [nwin][n] the opacity per unit length, dl [nwin] the
- * path length of one step.  work: 8 * 144 * 144 doubles per window.
+ * path length of one step.  work: 9 * 144 * 144 doubles per window.
  * Outputs T_gamma, T_electron [nwin][n][n] and the iteration counts.
  * Replaces the per-window host builds of the reference's cascade
  * (cascade/cascade.py:331-407, batched form 444-589). */

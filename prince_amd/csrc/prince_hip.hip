@@ -2116,8 +2116,8 @@ constexpr int CTILES = CTL * CTL;     // 81 output tiles
 constexpr int CTPW = (CTILES + CWAVES - 1) / CWAVES;  // tiles per wave (6)
 constexpr int64_t CMAT = (int64_t)CNP * CNP;
 constexpr int CKU = 4;                // k-steps whose loads are issued together (36 = 9 x 4)
-constexpr int CKU2 = 2;               // the same for the three-product pass (register budget)
-enum CascadeMat { CM_P = 0, CM_SG, CM_DE, CM_MIC, CM_G, CM_E, CM_E1, CM_ESC, CM_COUNT };
+constexpr int CKU2 = 2;               // the same for the 3 x 3-tile products (register budget)
+enum CascadeMat { CM_P = 0, CM_SG, CM_DE, CM_MIC, CM_G, CM_E, CM_E1, CM_ESC, CM_T, CM_COUNT };
 
 // rows of the contraction output per window (prh_cascade layout)
 __device__ __forceinline__ int64_t crow_emis(int) { return 0; }
@@ -2160,6 +2160,7 @@ __global__ void __launch_bounds__(CTHREADS) cascade_transfer_kernel(
   double* Ev = M + CM_E * CMAT;
   double* E1 = M + CM_E1 * CMAT;
   double* Esc = M + CM_ESC * CMAT;
+  double* Tc = M + CM_T * CMAT;  // G_c of the iteration, until it is added into G
 
   for (int i = tid; i < CNP; i += CTHREADS) {
     sE[i] = i < n ? cs.E[i] : 0.0;
@@ -2179,6 +2180,7 @@ __global__ void __launch_bounds__(CTHREADS) cascade_transfer_kernel(
     G[t] = (in && i == j) ? 1.0 : 0.0;
     Ev[t] = 0.0;
     Esc[t] = 0.0;
+    Tc[t] = 0.0;
   }
   __syncthreads();
   // per column j: pair-matrix energy normalisation, IC emission rate R_j
@@ -2251,79 +2253,122 @@ __global__ void __launch_bounds__(CTHREADS) cascade_transfer_kernel(
       for (int i = 0; i < n; ++i) Mic[(int64_t)i * CNP + j] = (Mic[(int64_t)i * CNP + j] / oe) * sE[j];
   }
   __syncthreads();
+  // the four fixed left operands of the iteration stored transposed (k-major):
+  // an operand fragment (16 rows x 4 k) is then four 128-byte row segments
+  double* fixed[4] = {P, Sg, De, Mic};
+  for (int m = 0; m < 4; ++m) {
+    double* X = fixed[m];
+    for (int64_t t = tid; t < CMAT; t += CTHREADS) Tc[t] = X[t];
+    __syncthreads();
+    for (int64_t t = tid; t < CMAT; t += CTHREADS) {
+      const int i = (int)(t / CNP), j = (int)(t % CNP);
+      X[t] = Tc[(int64_t)j * CNP + i];
+    }
+    __syncthreads();
+  }
 
   const int col = lane & 15;
   const int kq = lane >> 4;
   int it = 0;
   for (it = 1; it <= cs.max_scatter; ++it) {
     // e1 = e + P ((G - pesc G) dE);  esc += pesc G
+    // unit = 3 row tiles x 1 column tile (27 units): per k-step 3 + 1 operand
+    // loads for 3 products, the loads of CKU k-steps issued together
 #pragma unroll 1
-    for (int q = 0; q < CTPW; ++q) {
-      const int t = wv + CWAVES * q;
-      if (t >= CTILES) break;
-      const int r0 = (t / CTL) * 16, c0 = (t % CTL) * 16;
-      double4_t acc = {0.0, 0.0, 0.0, 0.0};
-      // the operands of CKU k-steps are loaded before their products
-      for (int k0 = 0; k0 < CNP; k0 += 4 * CKU) {
-        double a[CKU], g[CKU];
+    for (int u = wv; u < 3 * CTL; u += CWAVES) {
+      const int r0 = (u / CTL) * 48, c0 = (u % CTL) * 16;
+      double4_t acc[3];
 #pragma unroll
-        for (int u = 0; u < CKU; ++u) {
-          const int k = k0 + 4 * u + kq;
-          a[u] = P[(int64_t)(r0 + col) * CNP + k];
-          g[u] = G[(int64_t)k * CNP + c0 + col];
+      for (int r = 0; r < 3; ++r) acc[r] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+      for (int k0 = 0; k0 < CNP; k0 += 4 * CKU) {
+        double a[CKU][3], g[CKU];
+#pragma unroll
+        for (int v = 0; v < CKU; ++v) {
+          const int k = k0 + 4 * v + kq;
+#pragma unroll
+          for (int r = 0; r < 3; ++r) a[v][r] = P[(int64_t)k * CNP + r0 + 16 * r + col];
+          g[v] = G[(int64_t)k * CNP + c0 + col];
         }
 #pragma unroll
-        for (int u = 0; u < CKU; ++u) {
-          const int k = k0 + 4 * u + kq;
-          const double b = (g[u] - spesc[k] * g[u]) * sdE[k];
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b, acc, 0, 0, 0);
+        for (int v = 0; v < CKU; ++v) {
+          const int k = k0 + 4 * v + kq;
+          const double b = (g[v] - spesc[k] * g[v]) * sdE[k];
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+            acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[v][r], b, acc[r], 0, 0, 0);
         }
       }
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = r0 + kq + 4 * v, j = c0 + col;
-        const int64_t o = (int64_t)i * CNP + j;
-        E1[o] = Ev[o] + acc[v];
-        Esc[o] += spesc[i] * G[o];
+      for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int i = r0 + 16 * r + kq + 4 * v, j = c0 + col;
+          const int64_t o = (int64_t)i * CNP + j;
+          E1[o] = Ev[o] + acc[r][v];
+          Esc[o] += spesc[i] * G[o];
+        }
       }
     }
     __syncthreads();
-    // G_c = Mic (e1 cold dE), G_h = Sg (e1 hot dE), e = De (e1 hot dE)
+    // G_c = Mic (e1 cold dE) -> Tc, G_h = Sg (e1 hot dE) -> G, e = De (e1 hot dE)
+    // -> Ev; unit = one product on 3 x 3 tiles (27 units): per k-step 3 + 3
+    // operand loads for 9 products
 #pragma unroll 1
-    for (int q = 0; q < CTPW; ++q) {
-      const int t = wv + CWAVES * q;
-      if (t >= CTILES) break;
-      const int r0 = (t / CTL) * 16, c0 = (t % CTL) * 16;
-      double4_t gc = {0.0, 0.0, 0.0, 0.0}, gh = gc, en = gc;
+    for (int u = wv; u < 27; u += CWAVES) {
+      const int pr = u / 9, st = u % 9;
+      const int r0 = (st / 3) * 48, c0 = (st % 3) * 48;
+      const double* A = pr == 0 ? Mic : (pr == 1 ? Sg : De);
+      const bool cold = pr == 0;
+      double4_t acc[3][3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[r][c] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
       for (int k0 = 0; k0 < CNP; k0 += 4 * CKU2) {
-        double am[CKU2], as[CKU2], ad[CKU2], ek[CKU2];
+        double a[CKU2][3], e[CKU2][3];
 #pragma unroll
-        for (int u = 0; u < CKU2; ++u) {
-          const int k = k0 + 4 * u + kq;
-          const int64_t ra = (int64_t)(r0 + col) * CNP + k;
-          am[u] = Mic[ra];
-          as[u] = Sg[ra];
-          ad[u] = De[ra];
-          ek[u] = E1[(int64_t)k * CNP + c0 + col];
+        for (int v = 0; v < CKU2; ++v) {
+          const int k = k0 + 4 * v + kq;
+#pragma unroll
+          for (int r = 0; r < 3; ++r) a[v][r] = A[(int64_t)k * CNP + r0 + 16 * r + col];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) e[v][c] = E1[(int64_t)k * CNP + c0 + 16 * c + col];
         }
 #pragma unroll
-        for (int u = 0; u < CKU2; ++u) {
-          const int k = k0 + 4 * u + kq;
-          const double yc = scold[k] != 0.0 ? ek[u] * sdE[k] : 0.0;
-          const double yh = scold[k] != 0.0 ? 0.0 : ek[u] * sdE[k];
-          gc = __builtin_amdgcn_mfma_f64_16x16x4f64(am[u], yc, gc, 0, 0, 0);
-          gh = __builtin_amdgcn_mfma_f64_16x16x4f64(as[u], yh, gh, 0, 0, 0);
-          en = __builtin_amdgcn_mfma_f64_16x16x4f64(ad[u], yh, en, 0, 0, 0);
+        for (int v = 0; v < CKU2; ++v) {
+          const int k = k0 + 4 * v + kq;
+          const bool take = (scold[k] != 0.0) == cold;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const double y = take ? e[v][c] * sdE[k] : 0.0;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+              acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[v][r], y, acc[r][c], 0, 0, 0);
+          }
         }
       }
+      double* out = pr == 0 ? Tc : (pr == 1 ? G : Ev);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int i = r0 + kq + 4 * v, j = c0 + col;
-        const int64_t o = (int64_t)i * CNP + j;
-        Esc[o] += spesc[i] * gc[v];
-        G[o] = (gh[v] + gc[v]) - spesc[i] * gc[v];
-        Ev[o] = en[v];
+      for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int i = r0 + 16 * r + kq + 4 * v, j = c0 + 16 * c + col;
+            const int64_t o = (int64_t)i * CNP + j;
+            out[o] = acc[r][c][v];
+            if (cold) Esc[o] += spesc[i] * acc[r][c][v];
+          }
+        }
       }
+    }
+    __syncthreads();
+    // G = (G_h + G_c) - pesc G_c
+    for (int64_t t = tid; t < CMAT; t += CTHREADS) {
+      const int i = (int)(t / CNP);
+      G[t] = (G[t] + Tc[t]) - spesc[i] * Tc[t];
     }
     __syncthreads();
     // energy still in G and e per input column, relative to the column's input
@@ -2371,7 +2416,7 @@ __global__ void __launch_bounds__(CTHREADS) cascade_transfer_kernel(
       const int k = k0 + kq;
       const int j = c0 + col;
       const double a = Esc[(int64_t)(r0 + col) * CNP + k];
-      const double b = Mic[(int64_t)k * CNP + j] * sdE[j];
+      const double b = Mic[(int64_t)j * CNP + k] * sdE[j];  // Mic is stored transposed
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
 #pragma unroll
@@ -2762,7 +2807,7 @@ int32_t prh_cascade_transfer(const prh_cascade* cs, const double* contr, int64_t
   const int64_t rows = 2 * (int64_t)cs->n * cs->n + 2 * cs->n + 2;
   if (cstride < rows) return set_error("prh_cascade_transfer: cstride below the contraction rows");
   if (work_bytes < (int64_t)nwin * CM_COUNT * CMAT * 8)
-    return set_error("prh_cascade_transfer: workspace below 8 * 144 * 144 doubles per window");
+    return set_error("prh_cascade_transfer: workspace below 9 * 144 * 144 doubles per window");
   hipLaunchKernelGGL(cascade_transfer_kernel, dim3((unsigned)nwin), dim3(CTHREADS), 0,
                      (hipStream_t)stream, *cs, contr, cstride, dtau, dl, work, tg, te, iters);
   return check_hip(hipGetLastError(), "prh_cascade_transfer launch");

@@ -31,8 +31,8 @@ from ..model.cosmology import H
 from ..model.photonfields import PLANCK_PREF, CMBPhotonSpectrum, CombinedPhotonField, SyntheticEBL
 from ..model.units import PRINCE_UNITS
 
-#: bytes of per-window builder workspace (8 padded 144 x 144 FP64 matrices)
-WORK_PER_WINDOW = 8 * 144 * 144 * 8
+#: bytes of per-window builder workspace (9 padded 144 x 144 FP64 matrices)
+WORK_PER_WINDOW = 9 * 144 * 144 * 8
 
 
 def _torch():
