@@ -2110,10 +2110,10 @@ void launch_adjoint_two(const prh_operator& op, const prh_schedule& sch, const p
 // ---------------------------------------------------------------------------
 constexpr int CNP = 144;              // padded EM dimension (n <= 144)
 constexpr int CTL = CNP / 16;         // 9 tiles per matrix side
-constexpr int CWAVES = 16;
+constexpr int CWAVES = 8;
 constexpr int CTHREADS = CWAVES * WAVE;
 constexpr int CTILES = CTL * CTL;     // 81 output tiles
-constexpr int CTPW = (CTILES + CWAVES - 1) / CWAVES;  // tiles per wave (6)
+constexpr int CTPW = (CTILES + CWAVES - 1) / CWAVES;  // tiles per wave (11)
 constexpr int64_t CMAT = (int64_t)CNP * CNP;
 constexpr int CKU = 4;                // k-steps whose loads are issued together (36 = 9 x 4)
 constexpr int CKU2 = 2;               // the same for the 3 x 3-tile products (register budget)
