@@ -102,3 +102,24 @@ def test_noalloc_entry_point_rejects_before_touching_the_gpu():
     assert b"null" in lib.prh_last_error()
     rc = lib.prh_etd2_steps_noalloc(ctypes.byref(op), ctypes.byref(sch), 8, 0, 1, None)
     assert rc != 0 and b"bytes of scratch" in lib.prh_last_error()
+
+
+def test_uneven_step_rejects_bad_slots_and_live_counts_on_the_host():
+    """prh_uneven_step refuses a value-slot count other than 1 or 2 and a live
+    column count outside [0, K] before any launch."""
+    import ctypes
+
+    lib = _lib.load()
+    dummy = ctypes.c_int32(0)
+    ptr = ctypes.addressof(dummy)
+    op = _lib.PrhOperator(nspec=2, d=64, K=4, ntasks=16, tasks=ptr, kt=2, mvals_stride=128)
+    st = _lib.PrhState(x=ptr, a=ptr, fphi=ptr, step_ctl=ptr)
+    live = (ctypes.c_int32 * 2)(4, 5)
+    un = _lib.PrhUneven(nsteps=2, nwin=2, h=ptr, dldz=ptr, hadia=ptr, finj=ptr, win=ptr,
+                        rpair=ptr, mrhs=ptr, nslot=3, live=ctypes.addressof(live))
+    args = (ctypes.byref(op), ctypes.byref(un), ctypes.byref(st))
+    assert lib.prh_uneven_step(*args, 0, None) != 0
+    assert b"nslot" in lib.prh_last_error()
+    un.nslot = 2
+    assert lib.prh_uneven_step(*args, 1, None) != 0
+    assert b"live column count" in lib.prh_last_error()

@@ -110,3 +110,32 @@ def test_em_execute_restores_initial_photons():
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
     assert np.any(outs[0][1] != 0.0)
+
+
+def test_em_step_transfer_is_T_gamma_times_the_oracle_photon_step():
+    """qstep[t] = T_gamma(w_t) P(t), where P(t) is the oracle's photon ETD2 step
+    (etd2_step with the adiabatic FD operator, etd2.py:412-439) applied to the
+    unit vectors, at the first, second, middle and last steps of a solve."""
+    import torch
+
+    run = workloads.proton_em_run()
+    g = UHECRPropagationSolverETD2(0.05, 0.0, run)
+    g.add_source_class(workloads.proton_source(run))
+    plan = g.prepare(1e-3)
+    torch.cuda.synchronize()
+    Q = plan["em_bufs"]["qstep"].cpu().numpy()
+    Tg = plan["em_bufs"]["tg"].cpu().numpy()
+    zg, win = plan["z_grid"], np.asarray(plan["win"])
+    de = run.em_grid.d
+    o = OracleETD2(0.05, 0.0, run)
+    nst = len(zg) - 1
+    assert Q.shape == (nst, de, de)
+    worst = 0.0
+    for t in (0, 1, nst // 2, nst - 1):
+        em = {22: np.eye(de)}
+        o._em_step(zg[t], zg[t + 1] - zg[t], em)  # column j: the step of unit vector j
+        ref = Tg[win[t]] @ em[22]
+        dev = float(np.max(np.abs(Q[t] - ref)) / np.max(np.abs(ref)))
+        worst = max(worst, dev)
+    print(f"T_gamma P(t) against T_gamma times the oracle's photon step: {worst:.2e}")
+    assert worst < 1e-12
