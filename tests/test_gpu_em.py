@@ -130,12 +130,16 @@ def test_em_step_transfer_is_T_gamma_times_the_oracle_photon_step():
     o = OracleETD2(0.05, 0.0, run)
     nst = len(zg) - 1
     assert Q.shape == (nst, de, de)
-    worst = 0.0
+    worst = worst_step = 0.0
     for t in (0, 1, nst // 2, nst - 1):
         em = {22: np.eye(de)}
         o._em_step(zg[t], zg[t + 1] - zg[t], em)  # column j: the step of unit vector j
-        ref = Tg[win[t]] @ em[22]
+        T = Tg[win[t]]
+        ref = T @ em[22]
         dev = float(np.max(np.abs(Q[t] - ref)) / np.max(np.abs(ref)))
-        worst = max(worst, dev)
-    print(f"T_gamma P(t) against T_gamma times the oracle's photon step: {worst:.2e}")
-    assert worst < 1e-12
+        # the step's own contribution T (P - I), where a dropped or wrong term shows
+        step = float(np.max(np.abs((Q[t] - T) - (ref - T))) / np.max(np.abs(ref - T)))
+        worst, worst_step = max(worst, dev), max(worst_step, step)
+    print(f"T_gamma P(t) against T_gamma times the oracle's photon step: {worst:.2e} "
+          f"(step part {worst_step:.2e})")
+    assert worst < 1e-12 and worst_step < 1e-9
