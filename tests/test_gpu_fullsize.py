@@ -142,3 +142,38 @@ def test_off_diagonal_pattern_equals_oracle_split(photohad):
     assert isinstance(M_off, sp.csr_matrix)
     print(f"photohad={photohad}: {M_off.nnz} off-diagonal entries equal")
     s.close()
+
+
+def test_uneven_em_columns_against_oracle_over_many_windows():
+    """The uneven batch with the EM cascade over many refresh windows: columns
+    from z_max 1.3 and 0.7 (130 and 70 windows, each with its own EBL scale and
+    its own device-built cascade transfers) against oracle solves on CMB + s EBL;
+    two shorter columns retire early.  Nuclear states and photons <= 1e-8."""
+    from prince_amd.solvers.uneven import UnevenBatchSolverETD2, column_field
+
+    run = workloads.nuclear_em_run()
+    z_max = np.array([1.3, 0.7, 0.25, 0.12])
+    scale = np.array([0.7, 1.3, 1.1, 0.9])
+    K = z_max.size
+    srcs = workloads.batch_sources(run, K, seed=21)
+    g = UnevenBatchSolverETD2(z_max, run, ebl_scale=scale)
+    for k in range(K):
+        g[k].add_source_class(srcs[k])
+    g.solve(dz=1e-3)
+    n = run.dim_states
+    sl = run.em_slices()
+    base = run.photon_field
+    try:
+        for k in (0, 1):
+            run.photon_field = column_field(base, scale[k])
+            o = OracleETD2(float(z_max[k]), 0.0, run)
+            o.add_source_class(srcs[k])
+            o.solve(1e-3)
+            dn, nbn = rel_dev(g.state[:n, k], o.state)
+            dg, nbg = rel_dev(g.state[sl[22], k], o.em_state[22])
+            print(f"column {k}: z_max {z_max[k]} ({len(o.refresh_zs)} windows) scale {scale[k]}: "
+                  f"nuclear {dn:.3e} ({nbn} bins), photons {dg:.3e} ({nbg} bins)")
+            assert nbn > 1000 and nbg > 50
+            assert dn <= TOL and dg <= TOL
+    finally:
+        run.photon_field = base
