@@ -65,49 +65,68 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // Fold GEMM: out[w, r] = sum_j phot[w, j] * kern[r, j]
 // One wave owns 16 kernel rows and keeps their B fragments in registers for
 // the whole window loop; each 16-window tile is 16x16 outputs accumulated by
-// v_mfma_f64_16x16x4 over the photon axis.
+// v_mfma_f64_16x16x4 over the photon axis.  The 16 waves of a workgroup share
+// each block of 32 windows' photon rows through LDS (635 -> 510 us for the
+// 288 windows of the nuclear run: 0.31 -> 0.38 of FP64 MFMA peak).
 // Operand layout of the 16x16x4 f64 MFMA (lane l):
 //   A (16 x 4): A[l % 16][l / 16];  B (4 x 16): B[l / 16][l % 16];
 //   D (16 x 16): D[l / 16 + 4 * v][l % 16], v = 0..3 (rows interleaved by 4).
 // ---------------------------------------------------------------------------
 constexpr int FOLD_MAX_KSTEPS = 32;  // dph <= 128
 
-__global__ void __launch_bounds__(THREADS) fold_gemm_mfma(const double* __restrict__ kern,
-                                                          int64_t nrows, int dph,
-                                                          const double* __restrict__ phot,
-                                                          int nwin, double* __restrict__ out,
-                                                          int64_t out_stride) {
+constexpr int FOLD_WAVES = 16;  // waves sharing one staged window block
+constexpr int FOLD_THREADS = FOLD_WAVES * WAVE;
+
+__global__ void __launch_bounds__(FOLD_THREADS) fold_gemm_mfma(const double* __restrict__ kern,
+                                                               int64_t nrows, int dph,
+                                                               const double* __restrict__ phot,
+                                                               int nwin, double* __restrict__ out,
+                                                               int64_t out_stride) {
+  // the photon rows of the current 32 windows, staged in LDS once per
+  // workgroup (row stride dph + 1): the A fragments are LDS reads instead of
+  // L2 gathers, and two 16-window tiles run as independent MFMA chains
+  __shared__ double sp[32 * (4 * FOLD_MAX_KSTEPS + 1)];
   const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t wave = (int64_t)blockIdx.x * WAVES_PER_BLOCK + wave_id();
+  const int64_t wave = (int64_t)blockIdx.x * FOLD_WAVES + wave_id();
   const int64_t r0 = wave * 16;
-  if (r0 >= nrows) return;
+  const bool rows = r0 < nrows;  // waves past the last row still take part in the barriers
   const int col = lane & 15;  // row of the kernel tile / column of D
   const int kq = lane >> 4;   // 0..3
   const int64_t r = r0 + col;
   const int nk = (dph + 3) / 4;
+  const int ps = dph + 1;
 
   double bfrag[FOLD_MAX_KSTEPS];
 #pragma unroll
   for (int t = 0; t < FOLD_MAX_KSTEPS; ++t) {
     const int j = 4 * t + kq;
-    bfrag[t] = (t < nk && j < dph && r < nrows) ? kern[r * dph + j] : 0.0;
+    bfrag[t] = (rows && t < nk && j < dph && r < nrows) ? kern[r * dph + j] : 0.0;
   }
 
-  for (int w0 = 0; w0 < nwin; w0 += 16) {
-    double4_t acc = {0.0, 0.0, 0.0, 0.0};
-    const int wa = w0 + col;  // window row of the A fragment
+  for (int w0 = 0; w0 < nwin; w0 += 32) {
+    __syncthreads();  // the previous windows are consumed
+    for (int i = threadIdx.x; i < 32 * dph; i += FOLD_THREADS) {
+      const int wr = i / dph, j = i % dph;
+      sp[wr * ps + j] = w0 + wr < nwin ? phot[(int64_t)(w0 + wr) * dph + j] : 0.0;
+    }
+    __syncthreads();
+    if (!rows) continue;
+    double4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
 #pragma unroll
     for (int t = 0; t < FOLD_MAX_KSTEPS; ++t) {
       if (t < nk) {
         const int j = 4 * t + kq;
-        const double a = (wa < nwin && j < dph) ? phot[(int64_t)wa * dph + j] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bfrag[t], acc, 0, 0, 0);
+        const double a0 = j < dph ? sp[col * ps + j] : 0.0;
+        const double a1 = j < dph ? sp[(16 + col) * ps + j] : 0.0;
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bfrag[t], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bfrag[t], acc1, 0, 0, 0);
       }
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int w = w0 + kq + 4 * v;
-      if (w < nwin && r < nrows) out[(int64_t)w * out_stride + r] = acc[v];
+      if (w < nwin && r < nrows) out[(int64_t)w * out_stride + r] = acc0[v];
+      if (w + 16 < nwin && r < nrows) out[(int64_t)(w + 16) * out_stride + r] = acc1[v];
     }
   }
 }
@@ -1364,40 +1383,46 @@ __global__ void field_tables_kernel(const double* __restrict__ zs, const double*
 
 // Rate fold of listed columns: out[orow[c]][r] = sum_j kern[r][j] phot[irow[c]][j]
 // (the fold_gemm_mfma tiling with gathered input and scattered output rows).
-__global__ void __launch_bounds__(THREADS) fold_rows_mfma(const double* __restrict__ kern,
-                                                          int64_t nrows, int dph,
-                                                          const double* __restrict__ phot,
-                                                          const int32_t* __restrict__ irow,
-                                                          const int32_t* __restrict__ orow,
-                                                          int ncol, double* __restrict__ out,
-                                                          int64_t out_stride) {
+__global__ void __launch_bounds__(FOLD_THREADS) fold_rows_mfma(const double* __restrict__ kern,
+                                                               int64_t nrows, int dph,
+                                                               const double* __restrict__ phot,
+                                                               const int32_t* __restrict__ irow,
+                                                               const int32_t* __restrict__ orow,
+                                                               int ncol, double* __restrict__ out,
+                                                               int64_t out_stride) {
+  // the gathered photon rows of 32 listed columns at a time, staged in LDS
+  // once per workgroup as in fold_gemm_mfma; two 16-column MFMA chains per wave
+  __shared__ double sp[32 * (4 * FOLD_MAX_KSTEPS + 1)];
   const int lane = threadIdx.x & (WAVE - 1);
-  const int64_t wave = (int64_t)blockIdx.x * WAVES_PER_BLOCK + wave_id();
+  const int64_t wave = (int64_t)blockIdx.x * FOLD_WAVES + wave_id();
   const int64_t r0 = wave * 16;
-  if (r0 >= nrows) return;
+  const bool rows = r0 < nrows;  // waves past the last row still take part in the barriers
   const int col = lane & 15;
   const int kq = lane >> 4;
   const int64_t r = r0 + col;
   const int nk = (dph + 3) / 4;
+  const int ps = dph + 1;
   double bfrag[FOLD_MAX_KSTEPS];
 #pragma unroll
   for (int t = 0; t < FOLD_MAX_KSTEPS; ++t) {
     const int j = 4 * t + kq;
-    bfrag[t] = (t < nk && j < dph && r < nrows) ? kern[r * dph + j] : 0.0;
+    bfrag[t] = (rows && t < nk && j < dph && r < nrows) ? kern[r * dph + j] : 0.0;
   }
-  // two 16-column chunks per pass: two independent MFMA chains, and the photon
-  // rows of both gathered before either chain starts
   for (int c0 = 0; c0 < ncol; c0 += 32) {
+    __syncthreads();  // the previous columns are consumed
+    for (int i = threadIdx.x; i < 32 * dph; i += FOLD_THREADS) {
+      const int cr = i / dph, j = i % dph;
+      sp[cr * ps + j] = c0 + cr < ncol ? phot[(int64_t)irow[c0 + cr] * dph + j] : 0.0;
+    }
+    __syncthreads();
+    if (!rows) continue;
     double4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
-    const int ca0 = c0 + col, ca1 = c0 + 16 + col;
-    const int64_t prow0 = ca0 < ncol ? (int64_t)irow[ca0] * dph : 0;
-    const int64_t prow1 = ca1 < ncol ? (int64_t)irow[ca1] * dph : 0;
 #pragma unroll
     for (int t = 0; t < FOLD_MAX_KSTEPS; ++t) {
       if (t < nk) {
         const int j = 4 * t + kq;
-        const double a0 = (ca0 < ncol && j < dph) ? phot[prow0 + j] : 0.0;
-        const double a1 = (ca1 < ncol && j < dph) ? phot[prow1 + j] : 0.0;
+        const double a0 = j < dph ? sp[col * ps + j] : 0.0;
+        const double a1 = j < dph ? sp[(16 + col) * ps + j] : 0.0;
         acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bfrag[t], acc0, 0, 0, 0);
         acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bfrag[t], acc1, 0, 0, 0);
       }
@@ -2540,8 +2565,9 @@ int32_t prh_fold_gemm(const double* kern, int64_t nrows, int32_t dph, const doub
   const int64_t blocks = (waves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
   if (blocks > 0x7fffffff) return set_error("prh_fold_gemm: too many rows");
   if (dph <= 4 * FOLD_MAX_KSTEPS)
-    hipLaunchKernelGGL(fold_gemm_mfma, dim3((unsigned)blocks), dim3(THREADS), 0,
-                       (hipStream_t)stream, kern, nrows, (int)dph, phot, (int)nwin, out, out_stride);
+    hipLaunchKernelGGL(fold_gemm_mfma, dim3((unsigned)((waves + FOLD_WAVES - 1) / FOLD_WAVES)),
+                       dim3(FOLD_THREADS), 0, (hipStream_t)stream, kern, nrows, (int)dph, phot,
+                       (int)nwin, out, out_stride);
   else
     hipLaunchKernelGGL(contract_rows_mfma, dim3((unsigned)blocks), dim3(THREADS), 0,
                        (hipStream_t)stream, kern, nrows, (int)dph, phot, (int)nwin, out, out_stride);
@@ -2851,9 +2877,10 @@ int32_t prh_fold_rows(const double* kern, int64_t nrows, int32_t dph, const doub
   if (nrows == 0 || ncol == 0) return 0;
   if (!kern || !phot || !in_rows || !out_rows || !out) return set_error("prh_fold_rows: null pointer");
   const int64_t waves = (nrows + 15) / 16;
-  const int64_t blocks = (waves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
-  hipLaunchKernelGGL(fold_rows_mfma, dim3((unsigned)blocks), dim3(THREADS), 0, (hipStream_t)stream,
-                     kern, nrows, (int)dph, phot, in_rows, out_rows, (int)ncol, out, out_stride);
+  const int64_t blocks = (waves + FOLD_WAVES - 1) / FOLD_WAVES;
+  hipLaunchKernelGGL(fold_rows_mfma, dim3((unsigned)blocks), dim3(FOLD_THREADS), 0,
+                     (hipStream_t)stream, kern, nrows, (int)dph, phot, in_rows, out_rows, (int)ncol,
+                     out, out_stride);
   return check_hip(hipGetLastError(), "prh_fold_rows launch");
 }
 
