@@ -103,9 +103,18 @@ typedef struct prh_operator {
  * bits: 0 diagonal blocks only, 1 blocks of <= 8 diagonals, 2 general. */
 #define PRH_TASK_KIND_MASK 3
 #define PRH_TASK_TO_DPART 4 /* dense-row chunk task (see nfin) */
+/* Row-pair task (one-field stage kernels without dense-row modes): rows da
+ * and da+1, one part each, diagonal blocks only; blocks [b_begin, b_begin +
+ * (kind >> PRH_TASK_PAIR_SHIFT)) belong to row da, the rest up to b_end to
+ * row da+1.  The wave walks both rows' mothers merged, loading a mother panel
+ * both rows read once.  lead is the wave itself, nparts 1. */
+#define PRH_TASK_PAIR 8
+#define PRH_TASK_PAIR_SHIFT 8
 
 #define PRH_F_PARTIAL_DIFF 1 /* continuous-loss stencil term on */
 #define PRH_F_INJECTION 2    /* source term b(z) on */
+#define PRH_F_ROW_PAIRS 4    /* tasks hold row-pair tasks (PRH_TASK_PAIR): one-field
+                                forward stages only, no dense rows, kt <= 8 */
 
 typedef struct prh_schedule {
   int32_t nsteps;

@@ -44,8 +44,11 @@ SCRATCH_BYTES = 64 << 20
 
 PRH_F_PARTIAL_DIFF = 1
 PRH_F_INJECTION = 2
+PRH_F_ROW_PAIRS = 4
 PRH_TASK_KIND_MASK = 3
 PRH_TASK_TO_DPART = 4
+PRH_TASK_PAIR = 8
+PRH_TASK_PAIR_SHIFT = 8
 
 _p = ctypes.c_void_p
 _i32 = ctypes.c_int32

@@ -513,6 +513,83 @@ __device__ __forceinline__ void couple_part(const prh_operator& op, const double
   }
 }
 
+// Coupling product of two consecutive rows da, da+1 whose blocks are all
+// diagonal: blocks [b0, b1) of row da and [b1, b2) of row da+1, each in
+// mother order.  The walk merges the two mother lists, so a mother panel that
+// both rows read is loaded once and feeds both rows' FMAs; each row adds its
+// own blocks in the same order as a one-row walk (same bits).
+template <int KT>
+__device__ __forceinline__ void couple_pair(const prh_operator& op, const double* __restrict__ y,
+                                            const double* __restrict__ mv, int b0, int b1,
+                                            int b2, int k0, int lane, bool active,
+                                            double (&accA)[KT], double (&accB)[KT]) {
+  const int K = op.K;
+  const int d = op.d;
+  const int64_t Kd = (int64_t)K * d;
+  const int lb = (active ? lane : d - 1) * 8;
+  const auto rm = tile_rsrc(mv, (int)(op.mvals_stride * 8));
+  const int pbytes = (int)(Kd * 8);
+  int coff[KT];
+#pragma unroll
+  for (int kk = 0; kk < KT; ++kk) {
+    coff[kk] = (k0 + kk < K ? k0 + kk : K - 1) * d * 8;
+    accA[kk] = 0.0;
+    accB[kk] = 0.0;
+  }
+  // descriptors of the first 64 blocks of the pair, one per lane
+  int dmo = 0, dvo = 0;
+  if (b0 + lane < b2) {
+    const int* blk = op.blocks + 4 * (b0 + lane);
+    dmo = blk[0];
+    dvo = blk[3];
+  }
+  auto mother = [&](int b) -> int {
+    return b - b0 < WAVE ? __builtin_amdgcn_readlane(dmo, b - b0) : op.blocks[4 * b];
+  };
+  auto voff = [&](int b) -> int {
+    return b - b0 < WAVE ? __builtin_amdgcn_readlane(dvo, b - b0) : op.blocks[4 * b + 3];
+  };
+  constexpr int SPENT = 0x7fffffff;
+  constexpr int UB = Batch<KT, false>::DIAG;
+  int i = b0, j = b1;
+  while (i < b1 || j < b2) {
+    double mA[UB], mB[UB];
+    double xv[UB][KT];
+    bool uA[UB], uB[UB];
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      const int ma = i < b1 ? mother(i) : SPENT;
+      const int mb = j < b2 ? mother(j) : SPENT;
+      const int m = ma < mb ? ma : mb;
+      uA[u] = m != SPENT && ma == m;
+      uB[u] = m != SPENT && mb == m;
+      mA[u] = 0.0;
+      mB[u] = 0.0;
+      if (uA[u]) mA[u] = tload(rm, lb, voff(i) * 8);
+      if (uB[u]) mB[u] = tload(rm, lb, voff(j) * 8);
+      // a spent slot of the last batch reads the first block's panel (in
+      // bounds, unused)
+      const int mm = m != SPENT ? m : mother(b0);
+      const auto ry = tile_rsrc(y + (int64_t)mm * Kd, pbytes);
+#pragma unroll
+      for (int kk = 0; kk < KT; ++kk) xv[u][kk] = tload(ry, lb, coff[kk]);
+      i += uA[u] ? 1 : 0;
+      j += uB[u] ? 1 : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < UB; ++u) {
+      if (uA[u]) {
+#pragma unroll
+        for (int kk = 0; kk < KT; ++kk) accA[kk] += mA[u] * xv[u][kk];
+      }
+      if (uB[u]) {
+#pragma unroll
+        for (int kk = 0; kk < KT; ++kk) accB[kk] += mB[u] * xv[u][kk];
+      }
+    }
+  }
+}
+
 // Sum the parts of split rows through LDS, in part order.  Every wave of the
 // workgroup must call this (it contains the workgroup barrier).
 template <int KT>
@@ -646,128 +723,27 @@ __device__ __forceinline__ RowCoef row_coef_from(const RowIn& ri, double s, bool
 // dense rows' epilogue launch.
 enum DenseMode { DM_NONE = 0, DM_PASS = 1, DM_CHUNKS = 2, DM_FINISH = 3 };
 
-template <bool STAGE2, int KT, bool TWO, int WGW, int DM>
-__device__ __forceinline__ void stage_body(const prh_operator& op, const prh_schedule& sch,
-                                           const prh_state& st, const int* __restrict__ tasks,
-                                           int ntasks, int step_local, int use_ctl) {
-  __shared__ double red[WGW][KT][WAVE];
-  int step = step_local;
-  int ridx = -1;
-  if (use_ctl) {
-    step = st.step_ctl[0] + step_local;
-    if (step >= st.step_ctl[1]) return;  // uniform over the grid
-    ridx = step_local;
-  } else {
-    ridx = step - st.rec_first;
-  }
-  const bool record = st.rec_x != nullptr && ridx >= 0 && ridx < st.rec_count;
-  const int lane = threadIdx.x & (WAVE - 1);
-  // the two-field kernels with 8 or 16 right-hand sides per wave keep the
-  // vector form: with a scalar wave index the compiler pipelines their wider
-  // batches past the register budget (spills)
-  const int wv = (TWO && KT > 4) ? (int)(threadIdx.x / WAVE) : wave_id();
+// Epilogue of one row block da (bins = lanes) for the KT right-hand sides
+// k0 .. k0+KT-1: kappa, d, the phi factors, the FD stencil on kappa x, the
+// injection term and the stage update.  acc holds the coupling sums (the
+// second field already scaled in), xo / fo the row's own state and, in stage
+// 2, F(x) (loaded by the caller for the buffer-addressed kernels).
+template <bool STAGE2, int KT, bool TWO>
+__device__ __forceinline__ void stage_epilogue(const prh_operator& op, const prh_schedule& sch,
+                                               const prh_state& st, int step, int ridx,
+                                               bool record, int w, int da, int k0, int lane,
+                                               bool active, const double* __restrict__ xin,
+                                               const double (&acc)[KT], const double (&sc_)[KT],
+                                               const double (&xo)[KT], const double (&fo)[KT]) {
+  constexpr bool EBUF = !TWO || KT <= 4;
   const int K = op.K;
   const int d = op.d;
-  int gx, gy;
-  wg_coords(ntasks, K, KT, gx, gy, WGW);
-  const int k0 = gy * KT;
-  const int* tk = tasks + 6 * ((int64_t)gx * WGW + wv);
-  const int da = tk[0];
-  const bool active = lane < d;
-
-  const double* __restrict__ xin = STAGE2 ? st.a : st.x;
-  const int w = sch.win[step];
-  const double* __restrict__ mvC = op.mvals + (int64_t)w * op.mvals_stride;
-  const double* __restrict__ mvE = TWO ? op.mvals_e + (int64_t)w * op.mvals_stride : nullptr;
-
-  double acc[KT], accE[KT];
-  if (da >= 0) {
-    couple_part<KT, TWO>(op, xin, mvC, mvE, tk[1], tk[2], tk[5] & PRH_TASK_KIND_MASK, k0, lane,
-                         active, acc, accE);
-  } else {
-#pragma unroll
-    for (int kk = 0; kk < KT; ++kk) acc[kk] = accE[kk] = 0.0;
-  }
-  double sc_[KT];
-#pragma unroll
-  for (int kk = 0; kk < KT; ++kk) {
-    const int k = k0 + kk < K ? k0 + kk : K - 1;
-    sc_[kk] = TWO ? op.escale[k] : 0.0;
-    if (TWO) acc[kk] = acc[kk] + sc_[kk] * accE[kk];
-  }
-  if (DM == DM_FINISH && da >= 0) {
-    // dense row: this part's share of the mother-chunk partial sums, in chunk
-    // order (the parts' shares are then added in part order)
-    const int r = op.dense_row[da];
-    const int np = tk[4];
-    const int pi = wv - tk[3];
-    const int c1 = (int)((int64_t)(pi + 1) * op.ndchunk / np);
-#pragma unroll 4
-    for (int c = (int)((int64_t)pi * op.ndchunk / np); c < c1; ++c) {
-      const double* dp = st.dpart + ((int64_t)c * op.ndense + r) * K * d;
-#pragma unroll
-      for (int kk = 0; kk < KT; ++kk) {
-        const int k = k0 + kk;
-        acc[kk] += (k < K && active) ? dp[(int64_t)k * d + lane] : 0.0;
-      }
-    }
-  }
-  // one-field kernels: this row's [K][d] panels of the stage arrays as
-  // buffers (column offset in the scalar offset, bin offset per lane); a row's
-  // lead wave issues its own state (and, in stage 2, F(x)) for all KT columns
-  // before the part reduction so the loads overlap the wait
-  constexpr bool EBUF = !TWO || KT <= 4;
-  const bool erow = da >= 0 && wv == tk[3] && !(DM == DM_CHUNKS && (tk[5] & PRH_TASK_TO_DPART));
-  const int64_t rowoff = (int64_t)(da >= 0 ? da : 0) * K * d;
+  const int64_t rowoff = (int64_t)da * K * d;
   const int pb = (int)((int64_t)K * d * 8);
-  const int lb = (active ? lane : d - 1) * 8;
   const int lo = lane * 8;
-  const auto r_in = tile_rsrc(xin + rowoff, pb);
   const auto r_a = tile_rsrc(st.a + rowoff, pb);
   const auto r_f = tile_rsrc(st.fphi + rowoff, pb);
   const auto r_x = tile_rsrc(st.x + rowoff, pb);
-  double xo[KT], fo[KT];
-  if constexpr (EBUF) {
-    if (erow) {
-#pragma unroll
-      for (int kk = 0; kk < KT; ++kk) {
-        const int co = (k0 + kk < K ? k0 + kk : K - 1) * d * 8;
-        const double v = tload(r_in, lb, co);
-        xo[kk] = active ? v : 0.0;
-        fo[kk] = STAGE2 ? tload(r_f, lb, co) : 0.0;
-      }
-    }
-  }
-  reduce_parts<KT>(red, wv, lane, tk[3], tk[4], acc);
-  if (DM == DM_CHUNKS && da >= 0 && (tk[5] & PRH_TASK_TO_DPART)) {
-    // one mother chunk of a dense row: store its partial sum (chunk tk[3])
-    double* dp = st.dpart + ((int64_t)tk[3] * op.ndense + op.dense_row[da]) * K * d;
-#pragma unroll
-    for (int kk = 0; kk < KT; ++kk) {
-      const int k = k0 + kk;
-      if (k < K && active) dp[(int64_t)k * d + lane] = acc[kk];
-    }
-    return;
-  }
-  if (da < 0 || wv != tk[3]) return;
-  if (DM == DM_PASS) {
-    // dense rows: add the mother-chunk partial sums of dense_pass in chunk order
-    const int r = op.dense_row[da];
-    if (r >= 0) {
-#pragma unroll
-      for (int kk = 0; kk < KT; ++kk) {
-        const int k = k0 + kk;
-        if (k >= K || !active) continue;
-        double sum = 0.0;
-#pragma unroll 8
-        for (int c = 0; c < op.ndchunk; ++c)
-          sum += st.dpart[(((int64_t)c * op.ndense + r) * K + k) * d + lane];
-        acc[kk] = acc[kk] + sum;
-      }
-    }
-  }
-
-  // ---- epilogue: row block da, bins = lanes, KT right-hand sides
   const double dl = sch.dldz[step];
   const double hh = sch.h[step];
   const bool pdiff = (op.flags & PRH_F_PARTIAL_DIFF) != 0;
@@ -866,6 +842,177 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
       if (record) st.rec_fa[ridx * nk + own] = out;
     }
   }
+}
+
+// One stage for a row-pair task (one-field kernels without dense-row modes):
+// the two rows' coupling in one merged walk, both rows' own state (and F(x))
+// issued before the workgroup barrier every wave of the workgroup takes once,
+// then the two epilogues.
+template <bool STAGE2, int KT>
+__device__ __forceinline__ void stage_pair(const prh_operator& op, const prh_schedule& sch,
+                                           const prh_state& st, const int* __restrict__ tk,
+                                           int step, int ridx, bool record, int w, int k0,
+                                           int lane, bool active, const double* __restrict__ xin,
+                                           const double* __restrict__ mv) {
+  const int da = tk[0];
+  const int b0 = tk[1];
+  const int b1 = b0 + (tk[5] >> PRH_TASK_PAIR_SHIFT);
+  double accA[KT], accB[KT];
+  couple_pair<KT>(op, xin, mv, b0, b1, tk[2], k0, lane, active, accA, accB);
+  const int K = op.K;
+  const int d = op.d;
+  const int pb = (int)((int64_t)K * d * 8);
+  const int lb = (active ? lane : d - 1) * 8;
+  double xo[2][KT], fo[2][KT];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int64_t rowoff = (int64_t)(da + r) * K * d;
+    const auto r_in = tile_rsrc(xin + rowoff, pb);
+    const auto r_f = tile_rsrc(st.fphi + rowoff, pb);
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) {
+      const int co = (k0 + kk < K ? k0 + kk : K - 1) * d * 8;
+      const double v = tload(r_in, lb, co);
+      xo[r][kk] = active ? v : 0.0;
+      fo[r][kk] = STAGE2 ? tload(r_f, lb, co) : 0.0;
+    }
+  }
+  __syncthreads();  // the part-reduction barrier of the other waves
+  double sc0[KT];
+#pragma unroll
+  for (int kk = 0; kk < KT; ++kk) sc0[kk] = 0.0;
+  stage_epilogue<STAGE2, KT, false>(op, sch, st, step, ridx, record, w, da, k0, lane, active,
+                                    xin, accA, sc0, xo[0], fo[0]);
+  stage_epilogue<STAGE2, KT, false>(op, sch, st, step, ridx, record, w, da + 1, k0, lane, active,
+                                    xin, accB, sc0, xo[1], fo[1]);
+}
+
+template <bool STAGE2, int KT, bool TWO, int WGW, int DM>
+__device__ __forceinline__ void stage_body(const prh_operator& op, const prh_schedule& sch,
+                                           const prh_state& st, const int* __restrict__ tasks,
+                                           int ntasks, int step_local, int use_ctl) {
+  __shared__ double red[WGW][KT][WAVE];
+  int step = step_local;
+  int ridx = -1;
+  if (use_ctl) {
+    step = st.step_ctl[0] + step_local;
+    if (step >= st.step_ctl[1]) return;  // uniform over the grid
+    ridx = step_local;
+  } else {
+    ridx = step - st.rec_first;
+  }
+  const bool record = st.rec_x != nullptr && ridx >= 0 && ridx < st.rec_count;
+  const int lane = threadIdx.x & (WAVE - 1);
+  // the two-field kernels with 8 or 16 right-hand sides per wave keep the
+  // vector form: with a scalar wave index the compiler pipelines their wider
+  // batches past the register budget (spills)
+  const int wv = (TWO && KT > 4) ? (int)(threadIdx.x / WAVE) : wave_id();
+  const int K = op.K;
+  const int d = op.d;
+  int gx, gy;
+  wg_coords(ntasks, K, KT, gx, gy, WGW);
+  const int k0 = gy * KT;
+  const int* tk = tasks + 6 * ((int64_t)gx * WGW + wv);
+  const int da = tk[0];
+  const bool active = lane < d;
+
+  const double* __restrict__ xin = STAGE2 ? st.a : st.x;
+  const int w = sch.win[step];
+  const double* __restrict__ mvC = op.mvals + (int64_t)w * op.mvals_stride;
+  const double* __restrict__ mvE = TWO ? op.mvals_e + (int64_t)w * op.mvals_stride : nullptr;
+
+  if constexpr (!TWO && DM == DM_NONE && KT <= 8) {
+    if (da >= 0 && (tk[5] & PRH_TASK_PAIR)) {
+      stage_pair<STAGE2, KT>(op, sch, st, tk, step, ridx, record, w, k0, lane, active, xin, mvC);
+      return;
+    }
+  }
+  double acc[KT], accE[KT];
+  if (da >= 0) {
+    couple_part<KT, TWO>(op, xin, mvC, mvE, tk[1], tk[2], tk[5] & PRH_TASK_KIND_MASK, k0, lane,
+                         active, acc, accE);
+  } else {
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) acc[kk] = accE[kk] = 0.0;
+  }
+  double sc_[KT];
+#pragma unroll
+  for (int kk = 0; kk < KT; ++kk) {
+    const int k = k0 + kk < K ? k0 + kk : K - 1;
+    sc_[kk] = TWO ? op.escale[k] : 0.0;
+    if (TWO) acc[kk] = acc[kk] + sc_[kk] * accE[kk];
+  }
+  if (DM == DM_FINISH && da >= 0) {
+    // dense row: this part's share of the mother-chunk partial sums, in chunk
+    // order (the parts' shares are then added in part order)
+    const int r = op.dense_row[da];
+    const int np = tk[4];
+    const int pi = wv - tk[3];
+    const int c1 = (int)((int64_t)(pi + 1) * op.ndchunk / np);
+#pragma unroll 4
+    for (int c = (int)((int64_t)pi * op.ndchunk / np); c < c1; ++c) {
+      const double* dp = st.dpart + ((int64_t)c * op.ndense + r) * K * d;
+#pragma unroll
+      for (int kk = 0; kk < KT; ++kk) {
+        const int k = k0 + kk;
+        acc[kk] += (k < K && active) ? dp[(int64_t)k * d + lane] : 0.0;
+      }
+    }
+  }
+  // one-field kernels: this row's [K][d] panels of the stage arrays as
+  // buffers (column offset in the scalar offset, bin offset per lane); a row's
+  // lead wave issues its own state (and, in stage 2, F(x)) for all KT columns
+  // before the part reduction so the loads overlap the wait
+  constexpr bool EBUF = !TWO || KT <= 4;
+  const bool erow = da >= 0 && wv == tk[3] && !(DM == DM_CHUNKS && (tk[5] & PRH_TASK_TO_DPART));
+  const int64_t rowoff = (int64_t)(da >= 0 ? da : 0) * K * d;
+  const int pb = (int)((int64_t)K * d * 8);
+  const int lb = (active ? lane : d - 1) * 8;
+  const auto r_in = tile_rsrc(xin + rowoff, pb);
+  const auto r_f = tile_rsrc(st.fphi + rowoff, pb);
+  double xo[KT], fo[KT];
+  if constexpr (EBUF) {
+    if (erow) {
+#pragma unroll
+      for (int kk = 0; kk < KT; ++kk) {
+        const int co = (k0 + kk < K ? k0 + kk : K - 1) * d * 8;
+        const double v = tload(r_in, lb, co);
+        xo[kk] = active ? v : 0.0;
+        fo[kk] = STAGE2 ? tload(r_f, lb, co) : 0.0;
+      }
+    }
+  }
+  reduce_parts<KT>(red, wv, lane, tk[3], tk[4], acc);
+  if (DM == DM_CHUNKS && da >= 0 && (tk[5] & PRH_TASK_TO_DPART)) {
+    // one mother chunk of a dense row: store its partial sum (chunk tk[3])
+    double* dp = st.dpart + ((int64_t)tk[3] * op.ndense + op.dense_row[da]) * K * d;
+#pragma unroll
+    for (int kk = 0; kk < KT; ++kk) {
+      const int k = k0 + kk;
+      if (k < K && active) dp[(int64_t)k * d + lane] = acc[kk];
+    }
+    return;
+  }
+  if (da < 0 || wv != tk[3]) return;
+  if (DM == DM_PASS) {
+    // dense rows: add the mother-chunk partial sums of dense_pass in chunk order
+    const int r = op.dense_row[da];
+    if (r >= 0) {
+#pragma unroll
+      for (int kk = 0; kk < KT; ++kk) {
+        const int k = k0 + kk;
+        if (k >= K || !active) continue;
+        double sum = 0.0;
+#pragma unroll 8
+        for (int c = 0; c < op.ndchunk; ++c)
+          sum += st.dpart[(((int64_t)c * op.ndense + r) * K + k) * d + lane];
+        acc[kk] = acc[kk] + sum;
+      }
+    }
+  }
+
+  stage_epilogue<STAGE2, KT, TWO>(op, sch, st, step, ridx, record, w, da, k0, lane, active, xin,
+                                  acc, sc_, xo, fo);
 }
 
 template <bool STAGE2, int KT, bool TWO, int WGW, int DM>
@@ -1322,6 +1469,7 @@ int32_t check_uneven(const prh_operator* op, const prh_uneven* un, const prh_sta
   if (op->mvals_e) return set_error(w + ": one-field operators only");
   if (op->wg_waves == 8) return set_error(w + ": the uneven kernel runs 16-wave workgroups");
   if (op->ndense > 0) return set_error(w + ": the uneven kernel has no dense-row pass");
+  if (op->flags & PRH_F_ROW_PAIRS) return set_error(w + ": the uneven kernel has no row-pair tasks");
   if (!un->h || !un->dldz || !un->hadia || !un->finj || !un->win || !un->rpair || !un->mrhs)
     return set_error(w + ": null schedule buffer");
   if (un->nslot != 1 && un->nslot != 2) return set_error(w + ": nslot must be 1 or 2");
@@ -1985,6 +2133,11 @@ int32_t validate(const prh_operator* op, const prh_schedule* sch, const prh_stat
   if ((op->mvals_e == nullptr) != (op->escale == nullptr) ||
       (op->mvals_e != nullptr && op->rpair_e == nullptr))
     return set_error(std::string(who) + ": mvals_e, rpair_e and escale go together");
+  if ((op->flags & PRH_F_ROW_PAIRS) &&
+      (op->mvals_e || op->ndense > 0 || op->nfin != 0 || op->kt > 8 ||
+       std::string(who).rfind("prh_adjoint", 0) == 0))
+    return set_error(std::string(who) +
+                     ": row-pair tasks run in the one-field forward stages without dense rows, kt <= 8");
   return validate_em(op, st, who);
 }
 

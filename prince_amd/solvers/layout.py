@@ -154,41 +154,55 @@ def part_kind(blocks, b0, b1):
     return PART_GENERIC
 
 
-def build_tasks(layout: BlockLayout, part_cost=96, wg_waves=16):
+def build_tasks(layout: BlockLayout, part_cost=96, wg_waves=16, pair_rows=False):
     """Stage-kernel tasks [da, b_begin, b_end, lead, nparts, kind], wg_waves per workgroup.
 
     A row whose blocks cost more than ``part_cost`` is cut into up to
     ``wg_waves`` parts; all parts of a row are packed into consecutive waves
     of one workgroup (first fit, longest rows first) so they can be summed
-    through LDS.  Unused waves carry da = -1.
+    through LDS.  Unused waves carry da = -1.  ``pair_rows`` joins two
+    consecutive rows that are one part of diagonal blocks each into one
+    row-pair task (``TASK_PAIR``).
     """
     cost = block_cost(layout.blocks)
-    rows = []
+    groups = []
     for da in range(layout.nspec):
         b0, b1 = int(layout.brow_ptr[da]), int(layout.brow_ptr[da + 1])
         total = int(cost[b0:b1].sum()) if b1 > b0 else 0
         nparts = max(1, min(wg_waves, -(-total // part_cost), b1 - b0))
-        rows.append((da, split_row(cost, b0, b1, nparts)))
-    rows.sort(key=lambda r: -len(r[1]))
-    wgs = []
-    for da, parts in rows:
-        np_ = len(parts)
-        target = None
-        for g in wgs:
-            if wg_waves - len(g) >= np_:
-                target = g
-                break
-        if target is None:
-            target = []
-            wgs.append(target)
-        lead = len(target)
-        for b0, b1 in parts:
-            target.append([da, b0, b1, lead, np_, part_kind(layout.blocks, b0, b1)])
-    tasks = []
-    for g in wgs:
-        tasks.extend(g)
-        tasks.extend([[-1, 0, 0, 0, 0, 0]] * (wg_waves - len(g)))
-    return np.asarray(tasks, dtype=np.int32).reshape(-1, 6)
+        parts = split_row(cost, b0, b1, nparts)
+        groups.append([[da, p0, p1, 0, len(parts), part_kind(layout.blocks, p0, p1)]
+                       for p0, p1 in parts])
+    if pair_rows:
+        groups = pair_row_tasks(groups)
+    groups.sort(key=lambda g: -len(g))
+    return _pack_first_fit(groups, wg_waves)
+
+
+TASK_PAIR = 8  # kind flag of a row-pair task
+TASK_PAIR_SHIFT = 8  # row da's block count sits above this bit
+
+
+def pair_row_tasks(groups):
+    """Join consecutive one-part diagonal rows (groups in species order) two at a time."""
+
+    def single_diag(g):
+        return len(g) == 1 and g[0][5] == PART_DIAG
+
+    out = []
+    i = 0
+    while i < len(groups):
+        g = groups[i]
+        if i + 1 < len(groups) and single_diag(g) and single_diag(groups[i + 1]):
+            a, b = g[0], groups[i + 1][0]
+            assert b[0] == a[0] + 1 and b[1] == a[2]
+            out.append([[a[0], a[1], b[2], 0, 1,
+                         PART_DIAG | TASK_PAIR | ((a[2] - a[1]) << TASK_PAIR_SHIFT)]])
+            i += 2
+        else:
+            out.append(g)
+            i += 1
+    return out
 
 
 TASK_TO_DPART = 4  # kind flag of a dense-row chunk task

@@ -292,12 +292,13 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
     #: steps per captured hipGraph chunk
     graph_chunk = 64
 
-    def part_cost_for(self, K):
-        """Cost units per row part (a diagonal block costs 3, a band block 2 + nslot)."""
+    def part_cost_for(self, K, pairs=False):
+        """Cost units per row part (a diagonal block costs 3, a band block 2 +
+        nslot); 48 with row-pair tasks (K = 512: 562 vs 585 ms per solve)."""
         tuned = getattr(self, "_tuned", {}).get(("part_cost", K))
         if tuned is not None:
             return tuned
-        return 12 if K < 8 else 96
+        return 12 if K < 8 else (48 if pairs else 96)
 
     def wg_waves_for(self, K, with_adjoint=False):
         """Waves per stage workgroup: 8 from K = 64 on (more workgroups per CU:
@@ -335,6 +336,16 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         48)."""
         tuned = getattr(self, "_tuned", {}).get(("dense_chunk_cost", K))
         return tuned if tuned is not None else (24 if K < 8 else 96)
+
+    def pair_rows_for(self, K):
+        """Whether consecutive one-part diagonal rows run as row-pair tasks
+        (one wave walks both rows' mothers merged and loads a shared mother
+        panel once).  Only the one-field stages without dense-row modes take
+        them; elsewhere the request is ignored.
+        Default on from K = 256 (where the dense rows are ordinary split
+        rows): 616 -> 562 ms per K = 512 solve, bitwise the same state."""
+        tuned = getattr(self, "_tuned", {}).get(("pairs", K))
+        return bool(tuned) if tuned is not None else K >= 256
 
     def kt_for(self, K):
         """Right-hand sides per wave: 8 from K = 256 on (K = 512: 109 / 118 vs
@@ -638,7 +649,9 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         elif chunked is not None:
             tasks = chunked[0]
         else:
-            tasks = build_tasks(wl.fwd, self.part_cost_for(K), wgw)
+            pairs = (self.pair_rows_for(K) and not with_adjoint and not split_fields
+                     and self.kt_for(K) <= 8)
+            tasks = build_tasks(wl.fwd, self.part_cost_for(K, pairs), wgw, pair_rows=pairs)
         kt = self.kt_for(K)
         col_sources = self._column_sources()
         has_src = any(len(c) for c in col_sources) and self.enable_injection_jacobian
@@ -658,6 +671,8 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             flags |= _lib.PRH_F_PARTIAL_DIFF
         if has_src:
             flags |= _lib.PRH_F_INJECTION
+        if np.any(np.asarray(tasks)[:, 5] & _lib.PRH_TASK_PAIR):
+            flags |= _lib.PRH_F_ROW_PAIRS
         blocks = wl.blocks if dense is None else dense[3]
         blocks = blocks if blocks.size else np.zeros((1, 4))
         sshape = (nspec, K, d)
@@ -891,15 +906,18 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
     def tuning_candidates(self, K):
         """Stage configurations the autotuner times for K right-hand sides:
         right-hand sides per wave (kt), part budget (block cost per wave part,
-        i.e. rows per wave), waves per workgroup and the dense-row mode."""
+        i.e. rows per wave), waves per workgroup, the dense-row mode and, with
+        dense rows as split rows, row-pair tasks on or off."""
         kts = [kt for kt in (8, 4, 2, 1) if kt <= K][:2]
         out = []
         for kt in kts:
             for pc in (48, 96) if K >= 8 else (12, 24):
                 for wgw in (8, 16):
                     for dense in (("chunks", False) if K >= 8 else (True, "chunks", False)):
-                        out.append({("kt", K): kt, ("part_cost", K): pc,
-                                    ("wg_waves", K): wgw, ("dense", K): dense})
+                        for pairs in ((True, False) if dense is False else (False,)):
+                            out.append({("kt", K): kt, ("part_cost", K): pc,
+                                        ("wg_waves", K): wgw, ("dense", K): dense,
+                                        ("pairs", K): pairs})
         return out
 
     def autotune(self, dz=1e-3, steps=30, candidates=None, repeats=2):

@@ -23,10 +23,10 @@ TOL = 1e-8
 pytestmark = pytest.mark.gpu
 
 #: the stage configuration the benchmark's autotuner picks at K = 512 (8
-#: right-hand sides per wave, part budget 96, 8-wave workgroups, dense rows as
-#: ordinary split rows)
-BENCH_PLAN_512 = {("kt", 512): 8, ("part_cost", 512): 96, ("wg_waves", 512): 8,
-                  ("dense", 512): False}
+#: right-hand sides per wave, part budget 48, 8-wave workgroups, dense rows as
+#: ordinary split rows, row-pair tasks)
+BENCH_PLAN_512 = {("kt", 512): 8, ("part_cost", 512): 48, ("wg_waves", 512): 8,
+                  ("dense", 512): False, ("pairs", 512): True}
 
 
 def test_plain_nuclear_z3_against_fixture():
@@ -58,6 +58,7 @@ def test_batch512_bench_plan_against_fixture_columns():
     plan = s._plan
     assert plan["kt"] == 8 and plan["dense_mode"] is None
     assert plan["op"].wg_waves == 8 and plan["nwin"] == 288
+    assert int(np.sum((plan["tasks"][:, 5] & 8) != 0)) > 100  # row pairs
     worst = 0.0
     for i, k in enumerate(g["columns"]):
         dev, nb = rel_dev(s.state[:, k], g["state"][:, i])

@@ -166,6 +166,38 @@ def test_dense_rows_pass_matches_oracle_and_plain_tasks(mode):
     assert worst <= TOL and worst_oo <= TOL
 
 
+@pytest.mark.parametrize("K,kt,pc,wgw,zi,seed", [(512, 8, 48, 8, 0.1, 4), (37, 4, 96, 16, 0.15, 4),
+                                                (4, 2, 24, 8, 0.2, 1)])
+def test_row_pairs_are_bitwise_one_row_tasks(K, kt, pc, wgw, zi, seed):
+    """Row-pair tasks (two consecutive one-part diagonal rows per wave, their
+    mothers walked merged) give the same bits as one-row tasks: each row adds
+    its own blocks in the same order.  K = 512 with the batched benchmark's
+    plan, a ragged K = 37 (last column group short) and K = 4 with 2 columns
+    per wave; the K = 4 solve (the columns of the multi-RHS oracle test) is
+    also held against the oracle."""
+    run = workloads.nuclear_run()
+    srcs = workloads.batch_sources(run, K, seed=seed)
+    states = {}
+    for pairs in (False, True):
+        g = _multi(run, zi, K, srcs, {("kt", K): kt, ("part_cost", K): pc, ("wg_waves", K): wgw,
+                                      ("dense", K): False, ("pairs", K): pairs})
+        g.solve(dz=1e-3)
+        tasks = g._plan["tasks"]
+        npair = int(np.sum((tasks[:, 5] & 8) != 0))
+        assert (npair >= 40) == pairs, npair
+        states[pairs] = g.state.copy()
+        g.close()
+    assert np.array_equal(states[True], states[False])
+    if K <= 16:
+        r = OracleETD2(zi, 0.0, run, K=K)
+        for k in range(K):
+            r.add_source_class(srcs[k], k)
+        r.solve(dz=1e-3)
+        for k in range(K):
+            dev, nb = rel_dev(states[True][:, k], r.state[:, k])
+            assert nb > 100 and dev <= TOL, (k, dev)
+
+
 def test_autotuner_picks_a_candidate_and_keeps_results():
     run = workloads.nuclear_run()
     K = 64

@@ -13,14 +13,20 @@ from prince_amd import workloads
 from prince_amd.model.cosmology import H
 from prince_amd.model.partial_diff import single_block
 from prince_amd.model.units import PRINCE_UNITS
-from prince_amd.solvers.layout import build_block_layout, build_tasks, fd_tables
+from prince_amd.solvers.layout import (
+    TASK_PAIR,
+    TASK_PAIR_SHIFT,
+    build_block_layout,
+    build_tasks,
+    fd_tables,
+)
 from prince_amd.solvers.propagation import refresh_schedule, z_grid_for
 from prince_amd.solvers.run_inputs import pair_rate_per_energy, pair_species_scale
 
 from .parity import rel_dev
 
 
-def emulate(run, src, zi, zf, dz=1e-3, part_cost=24):
+def emulate(run, src, zi, zf, dz=1e-3, part_cost=24, pair_rows=False):
     ir = run.int_rates
     d = run.cr_grid.d
     nspec = run.spec_man.nspec
@@ -32,16 +38,32 @@ def emulate(run, src, zi, zf, dz=1e-3, part_cost=24):
     B = ir._batch_matrix
     m0 = B @ phot[0]
     lay = build_block_layout(ir.coupling_mat.indptr, ir.coupling_mat.indices, m0, nspec, d)
-    tasks = build_tasks(lay, part_cost)
+    tasks = build_tasks(lay, part_cost, pair_rows=pair_rows)
     assert lay.blocks.shape[0] == 0 or int((lay.blocks[:, 3] + lay.blocks[:, 2] * d).max()) <= lay.zslots
     assert tasks.shape[0] % 16 == 0
     assert tasks[:, 0].max() < nspec
-    assert set(tasks[tasks[:, 0] >= 0, 0]) == set(range(nspec))
-    # every row's parts are consecutive waves of one workgroup, in block order
+    is_pair = (tasks[:, 5] & TASK_PAIR) != 0
+    assert is_pair.any() == pair_rows
+    # every row has exactly one epilogue: a lead wave of its own or a pair
+    epilogues = []
+    for t, (da, b0, b1, lead, nparts, kind) in enumerate(tasks):
+        if da >= 0 and t % 16 == lead:
+            epilogues.extend([da, da + 1] if kind & TASK_PAIR else [da])
+    assert sorted(epilogues) == list(range(nspec))
+    # every row's parts are consecutive waves of one workgroup, in block order;
+    # a row pair is rows da, da + 1, one part of diagonal blocks each, split at
+    # b_begin + (kind >> TASK_PAIR_SHIFT), each row's mothers ascending
     for t, (da, b0, b1, lead, nparts, kind) in enumerate(tasks):
         if da < 0:
             continue
         wg, wv = divmod(t, 16)
+        if kind & TASK_PAIR:
+            bs = b0 + (kind >> TASK_PAIR_SHIFT)
+            assert lead == wv and nparts == 1
+            assert (b0, bs, b1) == tuple(int(v) for v in lay.brow_ptr[da:da + 3])
+            assert np.all(lay.blocks[b0:b1, 1] == 0) and np.all(lay.blocks[b0:b1, 2] == 1)
+            assert np.all(np.diff(lay.blocks[b0:bs, 0]) > 0) and np.all(np.diff(lay.blocks[bs:b1, 0]) > 0)
+            continue
         assert lead <= wv < lead + nparts <= 16
         first = wg * 16 + lead
         assert tasks[first, 1] == lay.brow_ptr[da]
@@ -81,6 +103,21 @@ def emulate(run, src, zi, zf, dz=1e-3, part_cost=24):
             for it in tasks:
                 da, b0, b1 = it[0], it[1], it[2]
                 if da < 0:
+                    continue
+                if it[5] & TASK_PAIR:
+                    # the merged walk: the smaller head mother, loaded once,
+                    # feeds every row whose head block it is
+                    bs = b0 + (it[5] >> TASK_PAIR_SHIFT)
+                    heads = {da: b0, da + 1: bs}
+                    ends = {da: bs, da + 1: b1}
+                    while any(heads[r] < ends[r] for r in heads):
+                        m = min(lay.blocks[heads[r], 0] for r in heads if heads[r] < ends[r])
+                        xv = xin[m]
+                        for r in heads:
+                            if heads[r] < ends[r] and lay.blocks[heads[r], 0] == m:
+                                voff = lay.blocks[heads[r], 3]
+                                acc_rows[r] += mv[voff:voff + d] * xv
+                                heads[r] += 1
                     continue
                 acc = np.zeros(d)
                 for b in range(b0, b1):
@@ -130,6 +167,18 @@ def test_emulated_layout_matches_oracle_nuclear():
     src = workloads.silicon_source(run)
     x = emulate(run, src, 0.012, 0.0, part_cost=48)
     ref = OracleETD2(0.012, 0.0, run)
+    ref.add_source_class(src)
+    ref.solve(1e-3)
+    dev, nb = rel_dev(x, ref.state)
+    assert nb > 100 and dev < 1e-10
+
+
+def test_emulated_row_pairs_match_oracle_nuclear():
+    """Row-pair tasks: the merged two-row walk over the host-built tasks."""
+    run = workloads.nuclear_run()
+    src = workloads.silicon_source(run)
+    x = emulate(run, src, 0.006, 0.0, part_cost=48, pair_rows=True)
+    ref = OracleETD2(0.006, 0.0, run)
     ref.add_source_class(src)
     ref.solve(1e-3)
     dev, nb = rel_dev(x, ref.state)
