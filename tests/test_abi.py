@@ -90,6 +90,34 @@ def test_abi_rejects_bad_operators_without_touching_the_gpu():
     assert b"16-wave" in lib.prh_last_error()
 
 
+def test_row_pair_tasks_are_refused_where_no_kernel_runs_them():
+    """An operator flagged PRH_F_ROW_PAIRS is refused on the host by the
+    two-field stages, the adjoint and the uneven batch (their kernels would run
+    a pair task as one long row)."""
+    import ctypes
+
+    lib = _lib.load()
+    dummy = ctypes.c_int32(0)
+    ptr = ctypes.addressof(dummy)
+    op = _lib.PrhOperator(nspec=2, d=64, K=96, ntasks=16, kt=1, flags=_lib.PRH_F_ROW_PAIRS,
+                          mvals_e=ptr, escale=ptr, rpair_e=ptr)
+    sch = _lib.PrhSchedule(nsteps=4)
+    st = _lib.PrhState(x=ptr, a=ptr, fphi=ptr, step_ctl=ptr)
+    assert lib.prh_etd2_steps(ctypes.byref(op), ctypes.byref(sch), ctypes.byref(st), 0, 1, None) != 0
+    assert b"row-pair" in lib.prh_last_error()
+    op.mvals_e = op.escale = op.rpair_e = None
+    adj = _lib.PrhAdjoint(lam=ptr, lam_a=ptr, u=ptr, v=ptr)
+    assert lib.prh_adjoint_step(ctypes.byref(op), ctypes.byref(sch), ctypes.byref(adj), 0, None) != 0
+    assert b"row-pair" in lib.prh_last_error()
+    op.kt = 2
+    op.mvals_stride = 128
+    op.tasks = ptr
+    un = _lib.PrhUneven(nsteps=2, nwin=2, h=ptr, dldz=ptr, hadia=ptr, finj=ptr, win=ptr,
+                        rpair=ptr, mrhs=ptr, nslot=2)
+    assert lib.prh_uneven_step(ctypes.byref(op), ctypes.byref(un), ctypes.byref(st), 0, None) != 0
+    assert b"row-pair" in lib.prh_last_error()
+
+
 def test_noalloc_entry_point_rejects_before_touching_the_gpu():
     """Null arguments and a scratch need beyond the static array are refused on
     the host, before any device call."""
