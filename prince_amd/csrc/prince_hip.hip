@@ -949,13 +949,26 @@ __device__ __forceinline__ void stage_body(const prh_operator& op, const prh_sch
     const int np = tk[4];
     const int pi = wv - tk[3];
     const int c1 = (int)((int64_t)(pi + 1) * op.ndchunk / np);
-#pragma unroll 4
-    for (int c = (int)((int64_t)pi * op.ndchunk / np); c < c1; ++c) {
-      const double* dp = st.dpart + ((int64_t)c * op.ndense + r) * K * d;
+    // the loads of CB chunks issued together, then added in chunk order
+    constexpr int CB = 16 / KT > 0 ? 16 / KT : 1;
+    for (int cb = (int)((int64_t)pi * op.ndchunk / np); cb < c1; cb += CB) {
+      double pv[CB][KT];
 #pragma unroll
-      for (int kk = 0; kk < KT; ++kk) {
-        const int k = k0 + kk;
-        acc[kk] += (k < K && active) ? dp[(int64_t)k * d + lane] : 0.0;
+      for (int u = 0; u < CB; ++u) {
+        const int c = cb + u < c1 ? cb + u : c1 - 1;
+        const double* dp = st.dpart + ((int64_t)c * op.ndense + r) * K * d;
+#pragma unroll
+        for (int kk = 0; kk < KT; ++kk) {
+          const int k = k0 + kk;
+          pv[u][kk] = (k < K && active) ? dp[(int64_t)k * d + lane] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < CB; ++u) {
+        if (cb + u < c1) {
+#pragma unroll
+          for (int kk = 0; kk < KT; ++kk) acc[kk] += pv[u][kk];
+        }
       }
     }
   }
