@@ -550,7 +550,10 @@ __device__ __forceinline__ void couple_pair(const prh_operator& op, const double
     return b - b0 < WAVE ? __builtin_amdgcn_readlane(dvo, b - b0) : op.blocks[4 * b + 3];
   };
   constexpr int SPENT = 0x7fffffff;
-  constexpr int UB = Batch<KT, false>::DIAG;
+  // merged steps whose loads are issued together: one more than a one-row
+  // walk at 8 right-hand sides (554 vs 563 ms per K = 512 solve; the one-row
+  // walk is slower with 3)
+  constexpr int UB = KT == 8 ? 3 : Batch<KT, false>::DIAG;
   int i = b0, j = b1;
   while (i < b1 || j < b2) {
     double mA[UB], mB[UB];
