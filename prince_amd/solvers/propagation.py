@@ -347,13 +347,32 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         tuned = getattr(self, "_tuned", {}).get(("pairs", K))
         return bool(tuned) if tuned is not None else K >= 256
 
-    def kt_for(self, K):
+    def adjoint_kt_for(self, K):
+        """Right-hand sides per wave of the adjoint (transposed) stages."""
+        tuned = getattr(self, "_tuned", {}).get(("kt_adjoint", K))
+        return tuned if tuned is not None else self.kt_for(K)
+
+    def adjoint_part_cost_for(self, K):
+        """Part budget of the transposed operator's tasks: 24 from K = 8 on (twice
+        the parts of the forward default; K = 64: backward 925 -> 850 ms, 12: 926)."""
+        tuned = getattr(self, "_tuned", {}).get(("part_cost_adjoint", K))
+        if tuned is not None:
+            return tuned
+        return self.part_cost_for(K) if K < 8 else 24
+
+    def kt_for(self, K, with_adjoint=False):
         """Right-hand sides per wave: 8 from K = 256 on (K = 512: 109 / 118 vs
         118 / 123 us per stage with 4), else the largest of 4, 2, 1 not above K
         (K = 64: 38 / 39 vs 48 / 52 us with 8: twice the workgroups)."""
         tuned = getattr(self, "_tuned", {}).get(("kt", K))
         if tuned is not None:
             return tuned
+        if with_adjoint and 2 <= K < 256:
+            # the two-field forward of an adjoint plan (16-wave workgroups) fills
+            # the chip better with 2: K = 64 forward 343 -> 286 ms, and the
+            # backward's chunk recompute with it (with 4 in the transposed
+            # stages: backward 850 -> 791 ms)
+            return 2
         for kt in ((8, 4, 2, 1) if K >= 256 else (4, 2, 1)):
             if kt <= K:
                 return kt
@@ -652,7 +671,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             pairs = (self.pair_rows_for(K) and not with_adjoint and not split_fields
                      and self.kt_for(K) <= 8)
             tasks = build_tasks(wl.fwd, self.part_cost_for(K, pairs), wgw, pair_rows=pairs)
-        kt = self.kt_for(K)
+        kt = self.kt_for(K, with_adjoint)
         col_sources = self._column_sources()
         has_src = any(len(c) for c in col_sources) and self.enable_injection_jacobian
         h, dldz, hadia, finj, coef, J = self._schedule_arrays(z_grid, col_sources)
@@ -716,7 +735,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             bufs["rpair_e"] = dv(rpair_e, np.float64)
             bufs["escale"] = dv(self._field_split[1], np.float64)
 
-        def make_op(tk, blk, dcf, wgw=16, dense_op=False):
+        def make_op(tk, blk, dcf, wgw=16, dense_op=False, kt=kt):
             return _lib.PrhOperator(
                 nspec=nspec, d=d, K=K, ntasks=tk.shape[0],
                 tasks=tk.data_ptr() if tk.numel() else None,
@@ -742,11 +761,12 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         op = make_op(bufs["tasks"], bufs["blocks"], bufs["dcoef"], wgw=wgw, dense_op=True)
         opT = None
         if with_adjoint:
-            tasks_t = build_tasks(wl.tr, self.part_cost_for(K), _lib.WG_WAVES)
+            tasks_t = build_tasks(wl.tr, self.adjoint_part_cost_for(K), _lib.WG_WAVES)
             bufs["tasks_t"] = dv(tasks_t, np.int32)
             bufs["blocks_t"] = dv(wl.blocks_t if wl.blocks_t.size else np.zeros((1, 4)), np.int32)
             bufs["dcoef_t"] = dv(fd_transposed_tables(self._fd_block), np.float64)
-            opT = make_op(bufs["tasks_t"], bufs["blocks_t"], bufs["dcoef_t"])
+            opT = make_op(bufs["tasks_t"], bufs["blocks_t"], bufs["dcoef_t"],
+                          kt=self.adjoint_kt_for(K))
         sch = _lib.PrhSchedule(
             nsteps=len(h), pad_=0, h=bufs["h"].data_ptr(), dldz=bufs["dldz"].data_ptr(),
             hadia=bufs["hadia"].data_ptr(), finj=bufs["finj"].data_ptr(),
