@@ -91,6 +91,26 @@ class CosmicRaySource:
     def evolution(self, z):
         return evolution_factor(self.source_evo_m, z)
 
+    def injection_coefficients(self, zs):
+        """``norm * evolution(z)`` for every z of ``zs`` (the per-step source
+        coefficients of a solve).  Point by point with the scalar arithmetic of
+        ``evolution`` (same bits); when a subclass keeps this class's
+        ``evolution``, the evolution law is looked up once instead of per z."""
+        zl = np.asarray(zs, dtype=np.float64).tolist()
+        n = self.norm
+        if type(self).evolution is CosmicRaySource.evolution:
+            m = self.source_evo_m
+            if any(z < 0 for z in zl):
+                raise ValueError("source evolution is defined for z >= 0 only")
+            if m == "flat":
+                return np.full(len(zl), n * 1.0)
+            if isinstance(m, float):
+                return np.array([n * _sfr_weighted(z, m) for z in zl], dtype=np.float64)
+            if isinstance(m, tuple) and len(m) == 2 and m[0] in EVOLUTIONS:
+                law, mm = EVOLUTIONS[m[0]], m[1]
+                return np.array([n * law(z, mm) for z in zl], dtype=np.float64)
+        return np.array([n * self.evolution(z) for z in zl], dtype=np.float64)
+
 
 class SimpleSource(CosmicRaySource):
     """norm E^-gamma exp(-E/Emax), params {pid: (gamma, Emax, norm)}."""

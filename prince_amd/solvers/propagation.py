@@ -52,6 +52,14 @@ from .run_inputs import (
 )
 
 
+
+def source_coefficients(src, zs):
+    """Per-step coefficients ``norm * evolution(z)`` of one source over ``zs``."""
+    f = getattr(src, "injection_coefficients", None)
+    if f is not None:
+        return f(zs)
+    return np.array([src.norm * src.evolution(z) for z in zs], dtype=np.float64)
+
 def z_grid_for(initial_z, final_z, dz):
     """Uniform z grid from initial_z to final_z (propagation.py:826-832)."""
     dz_step = -float(abs(dz))
@@ -602,7 +610,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         coef = np.zeros((nsteps, J, K))
         for kcol, srcs in enumerate(col_sources):
             for j, src in enumerate(srcs):
-                coef[:, j, kcol] = [src.norm * src.evolution(z) for z in z0]
+                coef[:, j, kcol] = source_coefficients(src, z0)
         return h, dldz, hadia, finj, coef, J
 
     def _source_tables(self, col_sources, J):

@@ -44,6 +44,7 @@ from .propagation import (
     _MultiRHSETD2SolverHIP,
     _torch,
     refresh_schedule,
+    source_coefficients,
     z_grid_for,
 )
 
@@ -175,7 +176,7 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
             finj[:n, c] = [v * 1.0 * PRINCE_UNITS.cm2sec for v in dl]
             gwin[:n, c] = off + win
             for j, src in enumerate(col_sources[c]):
-                coef[:n, j, c] = [src.norm * src.evolution(z) for z in z0]
+                coef[:n, j, c] = source_coefficients(src, z0)
             zw.extend(zs)
             sw.extend([self.ebl_scale[k]] * len(zs))
             firsts.append(off)

@@ -217,6 +217,25 @@ def test_sources_closed_forms():
         p.evolution(-0.1)
 
 
+def test_injection_coefficients_are_the_pointwise_evolution_bit_for_bit():
+    """The per-step coefficient table of a solve equals norm * evolution(z)
+    evaluated point by point, bit for bit, for every evolution law (at the
+    piecewise boundaries too) and for a subclass that overrides evolution."""
+    run = workloads.nuclear_run()
+    si = make_nucleus_pdg(28, 14)
+    zs = np.concatenate([np.linspace(0.0, 5.0, 1001), [0.97, 1.0, 4.48, 1.5]])
+    laws = ["flat", 2.3, ("SFR", 1.2), ("TDE", -1.0), ("simple", 2.7), ("simple_flat", 3.1),
+            ("simple_SFR", 2.2)]
+    srcs = [AugerFitSource(run, params={si: (1.0, 1e9, 1.0)}, norm=1.7, m=m) for m in laws]
+    srcs.append(BrokenEvolutionSource(run, params={si: (1.0, 1e9, 1.0)}, norm=0.3, m=3.0))
+    for src in srcs:
+        want = np.array([src.norm * src.evolution(z) for z in zs])
+        got = src.injection_coefficients(zs)
+        assert got.dtype == np.float64 and np.array_equal(got, want), src.source_evo_m
+    with pytest.raises(ValueError):
+        srcs[1].injection_coefficients([0.1, -0.1])
+
+
 def test_result_accessors_pure_species():
     run = workloads.nuclear_run()
     sm = run.spec_man
