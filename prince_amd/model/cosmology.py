@@ -7,6 +7,8 @@ rate used by float-valued source evolutions.
 
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 from . import config
@@ -15,6 +17,15 @@ from . import config
 def H(z, H0=config.H_0s):
     """Expansion rate in 1/s (cosmology.py:10-21)."""
     return H0 * np.sqrt(config.Omega_m * (1 + z) ** 3 + config.Omega_Lambda)
+
+
+def hubble_steps(zs, H0=config.H_0s):
+    """``H(z)`` for every z of ``zs``, point by point with the scalar
+    arithmetic of ``H`` (``math.sqrt`` is correctly rounded like numpy's) on
+    Python floats: the same bits without numpy's per-scalar dispatch."""
+    om, ol = config.Omega_m, config.Omega_Lambda
+    return [H0 * math.sqrt(om * (1 + z) ** 3 + ol)
+            for z in np.asarray(zs, dtype=np.float64).tolist()]
 
 
 def star_formation_rate(z, z_inhom=0.0):

@@ -28,7 +28,7 @@ import time
 import numpy as np
 
 from .. import _lib
-from ..model.cosmology import H
+from ..model.cosmology import H, hubble_steps
 from ..model.partial_diff import DifferentialOperator
 from ..model.units import PRINCE_UNITS
 from .layout import (
@@ -178,6 +178,19 @@ class UHECRPropagationSolver:
 
     def dldz(self, z):
         return -1.0 / ((1.0 + z) * H(z) * PRINCE_UNITS.cm2sec)
+
+    def step_rates(self, z0):
+        """``dldz(z)`` and ``H(z) cm2sec`` for every step start z of ``z0``,
+        point by point: bit for bit the scalar ``dldz`` and ``H`` (evaluated
+        on Python floats unless a subclass overrides ``dldz``)."""
+        c2s = PRINCE_UNITS.cm2sec
+        if type(self).dldz is UHECRPropagationSolver.dldz:
+            hs = hubble_steps(z0)
+            zl = np.asarray(z0, dtype=np.float64).tolist()
+            dl = [-1.0 / ((1.0 + z) * h * c2s) for z, h in zip(zl, hs)]
+            return np.array(dl, dtype=np.float64), np.array([h * c2s for h in hs], dtype=np.float64)
+        return (np.array([self.dldz(z) for z in z0], dtype=np.float64),
+                np.array([H(z) * c2s for z in z0], dtype=np.float64))
 
     def injection(self, dz, z):
         f = self.dldz(z) * dz * PRINCE_UNITS.cm2sec
@@ -599,12 +612,12 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         nsteps = len(z_grid) - 1
         z0 = z_grid[:-1]
         h = np.diff(z_grid)
-        dldz = np.array([self.dldz(z) for z in z0])
+        dldz, hub = self.step_rates(z0)
         if self.enable_adiabatic_losses and self.enable_partial_diff_jacobian:
-            hadia = np.array([H(z) * PRINCE_UNITS.cm2sec for z in z0])
+            hadia = hub
         else:
             hadia = np.zeros(nsteps)
-        finj = np.array([self.dldz(z) * 1.0 * PRINCE_UNITS.cm2sec for z in z0])
+        finj = dldz * 1.0 * PRINCE_UNITS.cm2sec
         J = max(1, max(len(c) for c in col_sources))
         K = len(col_sources)
         coef = np.zeros((nsteps, J, K))

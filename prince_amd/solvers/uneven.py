@@ -34,7 +34,7 @@ import numpy as np
 
 from .. import _lib
 from ..model import config as mconfig
-from ..model.cosmology import H
+from ..model.cosmology import hubble_steps
 from ..model.photonfields import PLANCK_PREF, CMBPhotonSpectrum, CombinedPhotonField, SyntheticEBL
 from ..model.units import PRINCE_UNITS
 from .layout import build_tasks
@@ -169,11 +169,11 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
             h[:n, c] = np.diff(zg)
             # scalar evaluation per step, as the plain path (numpy's vector pow and
             # sqrt round differently from the scalar ones)
-            dl = [self.dldz(z) for z in z0]
+            dl, hub = self.step_rates(z0)
             dldz[:n, c] = dl
             if self.enable_adiabatic_losses and self.enable_partial_diff_jacobian:
-                hadia[:n, c] = [H(z) * PRINCE_UNITS.cm2sec for z in z0]
-            finj[:n, c] = [v * 1.0 * PRINCE_UNITS.cm2sec for v in dl]
+                hadia[:n, c] = hub
+            finj[:n, c] = dl * 1.0 * PRINCE_UNITS.cm2sec
             gwin[:n, c] = off + win
             for j, src in enumerate(col_sources[c]):
                 coef[:n, j, c] = source_coefficients(src, z0)
@@ -307,7 +307,7 @@ class UnevenBatchSolverETD2(_MultiRHSETD2SolverHIP):
         t0 = time.perf_counter()
         tg, te, rbh, iters = cache.build(zw, run.photon_field, dz, scales=sw)
         E_p = run.cr_grid.grid
-        dl = PRINCE_UNITS.c * abs(dz) / ((1.0 + zw) * np.array([H(z) for z in zw]))
+        dl = PRINCE_UNITS.c * abs(dz) / ((1.0 + zw) * np.array(hubble_steps(zw)))
         dEoE = np.gradient(E_p) / E_p
         bhw = (torch.from_numpy(dl).to(dev)[:, None] * torch.from_numpy(dEoE).to(dev)[None, :]) * rpair
         pscale = pair_species_scale(self.pair_loss_rates_grid, self.spec_man)

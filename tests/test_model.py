@@ -236,6 +236,21 @@ def test_injection_coefficients_are_the_pointwise_evolution_bit_for_bit():
         srcs[1].injection_coefficients([0.1, -0.1])
 
 
+def test_step_rates_are_the_pointwise_dldz_and_hubble_bit_for_bit():
+    """The per-step tables (dl/dz and H cm2sec) a solve builds equal the scalar
+    dldz and H evaluated point by point, bit for bit."""
+    from prince_amd.model.cosmology import hubble_steps
+    from prince_amd.solvers.propagation import UHECRPropagationSolver, z_grid_for
+
+    solver = UHECRPropagationSolver.__new__(UHECRPropagationSolver)
+    for zmax in (0.013, 0.73, 3.0, 4.97):
+        z0 = z_grid_for(zmax, 0.0, 1e-3)[:-1]
+        assert np.array_equal(np.array(hubble_steps(z0)), np.array([H(z) for z in z0]))
+        dl, hub = solver.step_rates(z0)
+        assert np.array_equal(dl, np.array([solver.dldz(z) for z in z0]))
+        assert np.array_equal(hub, np.array([H(z) * PRINCE_UNITS.cm2sec for z in z0]))
+
+
 def test_result_accessors_pure_species():
     run = workloads.nuclear_run()
     sm = run.spec_man
