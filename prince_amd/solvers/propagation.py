@@ -321,16 +321,17 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             return tuned
         return 12 if K < 8 else (48 if pairs else 96)
 
-    def wg_waves_for(self, K, with_adjoint=False):
+    def wg_waves_for(self, K, with_adjoint=False, two_field=False):
         """Waves per stage workgroup: 8 from K = 64 on (more workgroups per CU:
         151 vs 165 us per stage at K = 512) and below K = 8 together with the
-        dense-row pass (plain nuclear solve 115 ms); 16 in between and for
+        dense-row pass (plain nuclear solve 115 ms); 16 in between, for
         adjoint plans (two-field forward at K = 64: 845 vs 1028 ms per
-        checkpointed solve)."""
+        checkpointed solve) and for two-field plans below K = 256 (K = 64 with
+        2 right-hand sides per wave: 285 vs 311 ms per solve)."""
         tuned = getattr(self, "_tuned", {}).get(("wg_waves", K))
         if tuned is not None:
             return tuned
-        if with_adjoint:
+        if with_adjoint or (two_field and K < 256):
             return 16
         return 8 if (K >= 64 or K < 8) else 16
 
@@ -381,16 +382,17 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             return tuned
         return self.part_cost_for(K) if K < 8 else 24
 
-    def kt_for(self, K, with_adjoint=False):
+    def kt_for(self, K, with_adjoint=False, two_field=False):
         """Right-hand sides per wave: 8 from K = 256 on (K = 512: 109 / 118 vs
         118 / 123 us per stage with 4), else the largest of 4, 2, 1 not above K
         (K = 64: 38 / 39 vs 48 / 52 us with 8: twice the workgroups)."""
         tuned = getattr(self, "_tuned", {}).get(("kt", K))
         if tuned is not None:
             return tuned
-        if with_adjoint and 2 <= K < 256:
-            # the two-field forward of an adjoint plan (16-wave workgroups) fills
-            # the chip better with 2: K = 64 forward 343 -> 286 ms, and the
+        if (with_adjoint or two_field) and 2 <= K < 256:
+            # two-field stages (16-wave workgroups) fill the chip better with 2:
+            # K = 64 two-field solve 371 -> 285 ms (4 and 8-wave workgroups
+            # before); an adjoint plan's forward 343 -> 286 ms, and the
             # backward's chunk recompute with it (with 4 in the transposed
             # stages: backward 850 -> 791 ms)
             return 2
@@ -668,9 +670,9 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
         t1 = time.perf_counter()
         if with_adjoint and self.enable_em_cascade:
             raise NotImplementedError("the adjoint sweep does not cover the EM cascade")
-        wgw = self.wg_waves_for(K, with_adjoint)
-        dense = chunked = None
         split_fields = self._split_fields() is not None
+        wgw = self.wg_waves_for(K, with_adjoint, split_fields)
+        dense = chunked = None
         mode = self.dense_rows_for(K)
         if mode and not with_adjoint and not split_fields and self.kt_for(K) <= 8:
             if mode == "pass":
@@ -692,7 +694,7 @@ class ETD2SolverHIP(UHECRPropagationSolverETD2):
             pairs = (self.pair_rows_for(K) and not with_adjoint and not split_fields
                      and self.kt_for(K) <= 8)
             tasks = build_tasks(wl.fwd, self.part_cost_for(K, pairs), wgw, pair_rows=pairs)
-        kt = self.kt_for(K, with_adjoint)
+        kt = self.kt_for(K, with_adjoint, split_fields)
         col_sources = self._column_sources()
         has_src = any(len(c) for c in col_sources) and self.enable_injection_jacobian
         h, dldz, hadia, finj, coef, J = self._schedule_arrays(z_grid, col_sources)
